@@ -1,0 +1,157 @@
+"""Headline benchmark: semi-implicit gradient-flow steps/s on one 256^3 grid
+(get_min_u_F, LPsrc.cpp:40-117) in FP64 and in mixed precision (FP32 fields
+and FFTs, FP64 sums), with the HBM roofline of the step and the CPU oracle
+timed beside it.
+
+A "step" is one iteration of get_min_u_F on the whole field: the pre-step
+energy E and ||grad F||, the update u_F' = (u_F + dt f_hat)/(1 + dt G), the
+residual max|Re(u_F' - u_F)|, and the new field phi' = IFFT(u_F').
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 256]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def algorithmic_bytes_per_step(N: int, s: int) -> int:
+    """B_step = 2 R s + 5 C 2s: r2c of f (phi in, f_hat out), the spectral
+    update (u_F and f_hat in, u_F' out), c2r (u_F' in, phi' out); R = N^3,
+    C = N^2 (N/2 + 1)."""
+    R = N ** 3
+    C = N * N * (N // 2 + 1)
+    return 2 * R * s + 5 * C * 2 * s
+
+
+def seed_field(seeds, N):
+    # icosahedral 30-vector star, |q| = 1, in a sphere of radius 6 (reference metric k = n/L)
+    return seeds.icosahedral_nucleus(N, L=26.0, kscale=1.0, amp=0.1, seed=0)
+
+
+def time_gpu(spm, torch, N, prec, steps, warmup, phi0):
+    g = spm.Grid(N, prec)
+    phi = torch.as_tensor(phi0).to(g.dtype).cuda()
+    total = warmup + steps
+    g.gf_begin(phi, total + 1, res_tol=-1.0, gn_tol=0.0)  # no early stop inside the timed region
+    g.gf_steps(warmup)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    g.gf_steps(steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1)
+    it, hist = g.gf_end(phi, total + 1)
+    assert it == total, (it, total)
+    assert np.all(np.isfinite(hist)), "non-finite energy in the timed steps"
+    g.close()
+    return dict(wall_s=wall, event_ms=ev_ms, hist=hist, ctx_bytes=g.ctx_bytes)
+
+
+def cpu_baseline(po, N, phi0, budget_s=20.0):
+    """The oracle (the reference's FP64 algorithm, OpenMP threads) on a bounded
+    sample of the same workload: gradient-flow steps at the same N."""
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    o = po.Oracle(N, nthreads=nthreads)
+    uR = phi0.astype(np.complex128)
+    uF = o.my_fft(uR)
+    o.get_min_u_F(uR, uF, 1, -1.0)  # warm-up (page-in of the work arrays)
+    n, t = 0, 0.0
+    while t < budget_s and n < 50:
+        t0 = time.perf_counter()
+        o.get_min_u_F(uR, uF, 1, -1.0)
+        t += time.perf_counter() - t0
+        n += 1
+    o.close()
+    return dict(value=n / t, unit="steps/s", cores=nthreads, kind="port",
+                sample="%d FP64 gradient-flow steps of the oracle at %d^3 (%.1f s)" % (n, N, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (the workload is one 256^3 field); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    spm = importlib.import_module("spm-iqc_amd")
+    seeds = importlib.import_module("spm-iqc_amd.seeds")
+    N = args.N
+    phi0 = seed_field(seeds, N)
+
+    r64 = time_gpu(spm, torch, N, spm.FP64, args.steps, args.warmup, phi0)
+    r32 = time_gpu(spm, torch, N, spm.FP32, args.steps, args.warmup, phi0)
+
+    sps64 = args.steps / (r64["event_ms"] / 1e3)
+    sps32 = args.steps / (r32["event_ms"] / 1e3)
+    B64 = algorithmic_bytes_per_step(N, 8)
+    B32 = algorithmic_bytes_per_step(N, 4)
+    ach64 = B64 * sps64
+    ach32 = B32 * sps32
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+
+        cpu = cpu_baseline(po, N, phi0, args.cpu_budget)
+
+    out = {
+        "metric": "gradient-flow steps/s at 256^3 FP64 vs mixed, % HBM roofline; p99 solve latency",
+        "value": round(sps64, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["event_ms"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded icosahedral nucleus, 30-vector star |q|=1, sphere r=6, tanh edge 1)",
+        "config": {"workload": "semi-implicit gradient flow (get_min_u_F), one %d^3 field, L=26, k=n/L, "
+                               "Tau=-0.0061, Gamma=0.30, dt=0.3" % N, "N": N, "batch": 1},
+        "mixed": {"dtype": "f32 fields/FFTs, f64 sums", "value": round(sps32, 3),
+                  "ms_per_step": round(r32["event_ms"] / args.steps, 4),
+                  "speedup_vs_fp64": round(sps32 / sps64, 3),
+                  "roofline_frac": round(ach32 / HBM_PEAK, 4)},
+        "roofline": {"bound": "hbm", "kernel": "gradient-flow step (5 launches: i-pass fwd+update+inv, "
+                                               "finalize, j-pass inv, k-pass c2r+f+r2c, j-pass fwd)",
+                     "achieved": round(ach64 / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(ach64 / HBM_PEAK, 4), "traffic": None,
+                     "algorithmic_bytes_per_step": B64},
+        "cpu_baseline": cpu,
+        "wall_s": {"fp64": round(r64["wall_s"], 4), "fp32": round(r32["wall_s"], 4)},
+        "energy_last": {"fp64": float(r64["hist"][-1, 0]), "fp32": float(r32["hist"][-1, 0])},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

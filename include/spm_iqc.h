@@ -1,0 +1,134 @@
+/*
+ * spm_iqc.h -- C ABI of the MI355X (gfx950) implementation of the
+ * pseudo-spectral Landau free-energy path of JKGroupCUDA/SPM-IQC.
+ *
+ * The reference has no FFI: its interface is the set of free C++ functions in
+ * include/*.h operating on process globals (Data.h:4-28).  This header is the
+ * plain-C boundary those functions map onto, one entry point per reference
+ * function (cited beside each).  The C++ layer with the reference's own
+ * signatures (spm_compat.h) is built on top of it.
+ *
+ * Conventions
+ *   - All field pointers are DEVICE pointers.  A real field is N^3 values of
+ *     the context precision (double for SPM_FP64, float for SPM_FP32) in the
+ *     reference's x-major order idx = (i*N + j)*N + k.  A half spectrum is
+ *     N*N*nhp complex values (interleaved re, im) indexed (i*N + j)*nhp + kz,
+ *     kz = 0..N/2, normalised by 1/N^3 like the reference's u_F_local.
+ *   - Every call is enqueued on the context's HIP stream.  Calls that return a
+ *     host scalar (double *) synchronise that stream.
+ *   - Return value: 0 on success, a negative SPM_E* code otherwise;
+ *     spm_last_error() gives a message.  (The reference has no error
+ *     reporting: a failure there crashes the process.)
+ *   - A context is not thread-safe; use one context per thread / request.
+ */
+#ifndef SPM_IQC_H
+#define SPM_IQC_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPM_FP64 0
+#define SPM_FP32 1
+
+#define SPM_OK 0
+#define SPM_EINVAL (-1)   /* bad argument (size, pointer, parameter) */
+#define SPM_ENOMEM (-2)   /* device allocation failed */
+#define SPM_EHIP (-3)     /* HIP runtime error */
+#define SPM_ENOTSUP (-4)  /* unsupported grid size or precision */
+
+typedef struct spm_ctx spm_ctx;
+
+/* Model and grid parameters: LPsystParameter (Initialization.cpp:7-42).
+ * kscale: reciprocal metric k_i = kscale * n_i / L; the reference hard-codes
+ * 1 (ProjMatrix = 1/L, Initialization.cpp:35-38). */
+typedef struct {
+    int N;          /* grid points per side: 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256 */
+    int precision;  /* SPM_FP64 or SPM_FP32 */
+    double L;       /* box length (26) */
+    double kscale;  /* 1.0 = the reference's metric; 2*pi = 2 pi n / L */
+    double Tau;     /* -0.0061 */
+    double Gamma;   /* 0.30 */
+    double dt;      /* gradient-flow step (0.3) */
+} spm_params;
+
+typedef struct {
+    int N, nh, nhp, precision;
+    long long R;          /* N^3 */
+    long long C;          /* N*N*nhp half-spectrum entries */
+    size_t real_bytes;    /* bytes of one real field */
+    size_t spec_bytes;    /* bytes of one half spectrum */
+    size_t ctx_bytes;     /* device bytes owned by the context */
+} spm_layout;
+
+/* --- context: LPsystParameter + memAllocation_local + get_ksquare_local +
+ *     get_epmckt2_local + get_ikt2_local + My_mpi_fft_set_up
+ *     (Initialization.cpp:7-174, My_FFT.cpp:41-98); spm_destroy replaces
+ *     Memfree_local_kspace / Memfree_local_data (Memfree.cpp:23-60). */
+int spm_create(const spm_params *p, spm_ctx **out);
+int spm_destroy(spm_ctx *c);
+/* Stream for all later calls (NULL = the legacy default stream).  A new
+ * context uses a non-blocking stream of its own. */
+int spm_set_stream(spm_ctx *c, void *hip_stream);
+int spm_get_layout(const spm_ctx *c, spm_layout *out);
+const char *spm_last_error(void);
+
+/* get_GK (Initialization.cpp:150-166), host function */
+double spm_get_GK(double k2);
+
+/* --- My_FFT (My_FFT.cpp:4-21): c2c on the reference's full complex layout
+ *     (N^3 interleaved doubles/floats); forward normalised by 1/N^3, inverse
+ *     unnormalised.  in may equal out. */
+int spm_my_fft(spm_ctx *c, const void *x_R, void *x_F);
+int spm_my_ifft(spm_ctx *c, const void *x_F, void *x_R);
+
+/* Real-field transforms on the half-spectrum layout (the forms the path
+ * uses internally): S = FFT(x)/N^3; x = IFFT(S), S preserved. */
+int spm_r2c(spm_ctx *c, const void *x, void *S);
+int spm_c2r(spm_ctx *c, const void *S, void *x);
+
+/* --- LPsrc */
+/* get_Ene (LPsrc.cpp:8-38) of the real field phi (u_F = FFT(phi)/N^3). */
+int spm_energy(spm_ctx *c, const void *phi, double *E);
+/* ngrad_cam (LPsrc.cpp:119-173): grad = -dF/dphi; *nrm (may be NULL)
+ * receives nrmp(grad) (BasicOperators.cpp:158-164). */
+int spm_ngrad(spm_ctx *c, const void *phi, void *grad, double *nrm);
+/* get_min_u_F (LPsrc.cpp:40-117) on phi, in place.  Iterates while
+ * res > res_tol and ||grad F|| > gn_tol (gn_tol <= 0 disables it) and fewer
+ * than max_iter iterations (the reference: res_tol 1e-16, max_iter 100000).
+ * hist (host, may be NULL) receives 3 doubles per iteration: E and
+ * ||grad F|| of the pre-step field and the residual max|Re(u_F' - u_F)|.
+ * check_every: iterations queued between host convergence checks (>= 1).
+ * *iters receives the iteration count. */
+int spm_gradflow(spm_ctx *c, void *phi, int max_iter, double res_tol, double gn_tol, int check_every,
+                 double *hist, int *iters);
+
+/* The same loop in three asynchronous parts, for callers that drive the
+ * iterations themselves (graph capture, timing): begin = set-up of
+ * get_min_u_F's state from phi; steps = enqueue nsteps iterations (those
+ * past the stop condition do nothing); end = write the final field to phi
+ * (if not NULL), synchronise, return the history as spm_gradflow does. */
+int spm_gf_begin(spm_ctx *c, const void *phi, int max_iter, double res_tol, double gn_tol);
+int spm_gf_steps(spm_ctx *c, int nsteps);
+int spm_gf_end(spm_ctx *c, void *phi, double *hist, int *iters);
+
+/* --- LOBPCG */
+/* hv_cam (LOBPCG.cpp:113-166): hv = Hessian(phi) v minus its mean. */
+int spm_hv(spm_ctx *c, const void *phi, const void *v, void *hv);
+/* precond_camnew (LOBPCG.cpp:168-199), in place. */
+int spm_precond(spm_ctx *c, void *w);
+
+/* --- BasicOperators: inpfp / nrmp (BasicOperators.cpp:132-172), FP64 sums. */
+int spm_inpfp(spm_ctx *c, const void *p, const void *q, double *out);
+int spm_nrmp(spm_ctx *c, const void *p, double *out);
+/* FuncsLinear2Cplx (BasicOperators.cpp:17-26) on real fields: r = a1 F1 + a2 F2 */
+int spm_lin2(spm_ctx *c, void *r, double a1, const void *F1, double a2, const void *F2);
+
+/* --- compat helpers: real part of a full complex field <-> real field */
+int spm_cplx_to_real(spm_ctx *c, const void *z, void *x);
+int spm_real_to_cplx(spm_ctx *c, const void *x, void *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

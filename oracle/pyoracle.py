@@ -1,0 +1,227 @@
+"""ctypes binding of the CPU FP64 oracle (lp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the ``cpu_baseline`` leg of bench.py, as the checker.  The product path
+(spm-iqc_amd/) never imports it.
+
+Fields are numpy complex128 arrays of shape (N, N, N) (layout-compatible with
+the reference's ``fftw_complex`` = ``double[2]``, x-major); only the real part
+of a real field is meaningful, as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblp_oracle.so")
+_lib = None
+
+_c = ctypes.c_void_p
+_d = ctypes.c_double
+_i = ctypes.c_int
+_P = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "lpo_create": (_c, [_i, _d, _d, _d, _d, _d, _i]),
+            "lpo_destroy": (None, [_c]),
+            "lpo_get_GK": (_d, [_d]),
+            "lpo_tables": (None, [_c, _c, _c, _c]),
+            "lpo_my_fft": (None, [_c, _c, _c]),
+            "lpo_my_ifft": (None, [_c, _c, _c]),
+            "lpo_inpfp": (_d, [_c, _c, _c]),
+            "lpo_nrmp": (_d, [_c, _c]),
+            "lpo_mean": (_d, [_c, _c]),
+            "lpo_get_Ene": (_d, [_c, _c, _c]),
+            "lpo_get_min_u_F": (_i, [_c, _c, _c, _i, _d, _c, _c, _c]),
+            "lpo_ngrad_cam": (None, [_c, _c, _c]),
+            "lpo_hv_cam": (None, [_c, _c, _c, _c]),
+            "lpo_precond_camnew": (None, [_c, _c]),
+            "lpo_rayleighq": (_d, [_c, _c, _c]),
+            "lpo_lobpcg_initialization": (None, [_c, _i, _c, _c]),
+            "lpo_lobpcg_inip": (_i, [_c, _c, _c, _i, _i, _c]),
+            "lpo_get_Tv": (None, [_c, _c, _c, _c]),
+            "lpo_Natural_Forces": (None, [_c, _c, _c]),
+            "lpo_MinMax_Forces": (None, [_c, _c, _c]),
+            "lpo_Spring_Pairing": (_i, [_c, _c, _c, _i, _d, _c]),
+            "lpo_sym_eig": (None, [_i, _c, _c, _c]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def get_GK(k2: float) -> float:
+    """Initialization.cpp:150-166."""
+    return lib().lpo_get_GK(float(k2))
+
+
+def sym_eig(A: np.ndarray):
+    A = np.array(A, dtype=np.float64, order="C", copy=True)
+    n = A.shape[0]
+    w = np.empty(n)
+    Z = np.empty((n, n))
+    lib().lpo_sym_eig(n, _p(A), _p(w), _p(Z))
+    return w, Z
+
+
+class Oracle:
+    """One reference 'process' holding the whole N^3 grid (numprocs = 1)."""
+
+    def __init__(self, N, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, nthreads=0):
+        self.N = int(N)
+        self.R = self.N ** 3
+        self.params = dict(L=L, kscale=kscale, Tau=Tau, Gamma=Gamma, dt=dt)
+        self.h = lib().lpo_create(self.N, L, kscale, Tau, Gamma, dt, int(nthreads))
+        if not self.h:
+            raise ValueError("oracle: bad grid size %r" % (N,))
+
+    def close(self):
+        if self.h:
+            lib().lpo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- helpers -------------------------------------------------------
+    def field(self, real=None) -> np.ndarray:
+        """A reference-layout complex field; real part from `real` if given."""
+        z = np.zeros((self.N,) * 3, dtype=np.complex128)
+        if real is not None:
+            z.real = np.asarray(real, dtype=np.float64).reshape(z.shape)
+        return z
+
+    def _c(self, a) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.size % self.R == 0
+        return a
+
+    def tables(self):
+        ksq, G, ikt2 = (np.empty((self.N,) * 3) for _ in range(3))
+        lib().lpo_tables(self.h, _p(ksq), _p(G), _p(ikt2))
+        return ksq, G, ikt2
+
+    # --- My_FFT ----------------------------------------------------------
+    def my_fft(self, x):
+        x = self._c(x)
+        y = np.empty_like(x)
+        lib().lpo_my_fft(self.h, _p(x), _p(y))
+        return y
+
+    def my_ifft(self, x):
+        x = self._c(x)
+        y = np.empty_like(x)
+        lib().lpo_my_ifft(self.h, _p(x), _p(y))
+        return y
+
+    # --- BasicOperators ------------------------------------------------
+    def inpfp(self, p, q):
+        return lib().lpo_inpfp(self.h, _p(self._c(p)), _p(self._c(q)))
+
+    def nrmp(self, p):
+        return lib().lpo_nrmp(self.h, _p(self._c(p)))
+
+    # --- LPsrc -----------------------------------------------------------
+    def get_Ene(self, u_R, u_F):
+        return lib().lpo_get_Ene(self.h, _p(self._c(u_R)), _p(self._c(u_F)))
+
+    def energy_of(self, phi_real):
+        """E of a real field: u_F = my_fft(phi), then get_Ene."""
+        u_R = self.field(phi_real)
+        return self.get_Ene(u_R, self.my_fft(u_R))
+
+    def get_min_u_F(self, u_R, u_F, max_iter, res_tol=1e-16, gnorm=False):
+        """In place on (u_R, u_F); returns (iters, E[iters], res[iters], gnorm|None)."""
+        assert u_R.flags["C_CONTIGUOUS"] and u_F.flags["C_CONTIGUOUS"]
+        E = np.zeros(max_iter)
+        res = np.zeros(max_iter)
+        g = np.zeros(max_iter) if gnorm else None
+        it = lib().lpo_get_min_u_F(self.h, _p(u_R), _p(u_F), int(max_iter), float(res_tol),
+                                   _p(E), _p(res), _p(g) if gnorm else None)
+        return it, E[:it], res[:it], (g[:it] if gnorm else None)
+
+    def ngrad_cam(self, phir):
+        phir = self._c(phir)
+        g = np.empty_like(phir)
+        lib().lpo_ngrad_cam(self.h, _p(phir), _p(g))
+        return g
+
+    # --- LOBPCG ----------------------------------------------------------
+    def hv_cam(self, phir, v):
+        phir, v = self._c(phir), self._c(v)
+        hv = np.empty_like(v)
+        lib().lpo_hv_cam(self.h, _p(phir), _p(v), _p(hv))
+        return hv
+
+    def precond_camnew(self, w):
+        w = np.array(self._c(w), copy=True)
+        lib().lpo_precond_camnew(self.h, _p(w))
+        return w
+
+    def rayleighq(self, x, v):
+        return lib().lpo_rayleighq(self.h, _p(self._c(x)), _p(self._c(v)))
+
+    def lobpcg_initialization(self, start):
+        """start: (k, N, N, N) float64 random block; returns V (k, N, N, N) complex."""
+        start = np.ascontiguousarray(start, dtype=np.float64)
+        k = start.shape[0]
+        V = np.zeros((k,) + (self.N,) * 3, dtype=np.complex128)
+        lib().lpo_lobpcg_initialization(self.h, k, _p(start), _p(V))
+        return V
+
+    def lobpcg_inip(self, x, V, maxiter=21):
+        """In place on V (k, N, N, N); returns the last lowest-k Ritz values."""
+        assert V.flags["C_CONTIGUOUS"] and V.dtype == np.complex128
+        k = V.shape[0]
+        ritz = np.zeros(k)
+        lib().lpo_lobpcg_inip(self.h, _p(self._c(x)), _p(V), k, int(maxiter), _p(ritz))
+        return ritz
+
+    # --- SPM ---------------------------------------------------------------
+    def get_Tv(self, Phi0, Phi1):
+        v = np.empty_like(self._c(Phi0))
+        lib().lpo_get_Tv(self.h, _p(self._c(Phi0)), _p(self._c(Phi1)), _p(v))
+        return v
+
+    def Natural_Forces(self, x, v):
+        """In place on x."""
+        assert x.flags["C_CONTIGUOUS"] and x.dtype == np.complex128
+        lib().lpo_Natural_Forces(self.h, _p(x), _p(self._c(v)))
+
+    def MinMax_Forces(self, x, v):
+        assert x.flags["C_CONTIGUOUS"] and x.dtype == np.complex128
+        lib().lpo_MinMax_Forces(self.h, _p(x), _p(self._c(v)))
+
+    def Spring_Pairing(self, Phi0, Phi1, maxiter, tol):
+        """In place on Phi0, Phi1; returns (iters, last residual)."""
+        r = np.zeros(1)
+        it = lib().lpo_Spring_Pairing(self.h, _p(Phi0), _p(Phi1), int(maxiter), float(tol), _p(r))
+        return it, float(r[0])

@@ -1,0 +1,296 @@
+"""spm-iqc_amd: MI355X (gfx950) implementation of the pseudo-spectral Landau
+free-energy path of JKGroupCUDA/SPM-IQC.
+
+The compute path is the HIP library ``lib/libspm_iqc.so`` behind the C ABI
+declared in ``include/spm_iqc.h``; this module binds it with ctypes.  PyTorch
+provides device memory and the stream (every call is enqueued on torch's
+current stream).  There is no CPU fallback: if the library is missing, or no
+GPU is present, the calls raise.
+
+Import it by path, e.g. ``importlib.import_module("spm-iqc_amd")`` with the
+repository root on ``sys.path``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libspm_iqc.so")
+
+FP64 = 0
+FP32 = 1
+SUPPORTED_N = (4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256)
+
+# LPsystParameter (Initialization.cpp:7-42)
+REFERENCE_PARAMS = dict(L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3)
+
+_lib = None
+
+
+class SpmError(RuntimeError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("precision", ctypes.c_int), ("L", ctypes.c_double),
+                ("kscale", ctypes.c_double), ("Tau", ctypes.c_double), ("Gamma", ctypes.c_double),
+                ("dt", ctypes.c_double)]
+
+
+class _Layout(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("nh", ctypes.c_int), ("nhp", ctypes.c_int), ("precision", ctypes.c_int),
+                ("R", ctypes.c_longlong), ("C", ctypes.c_longlong), ("real_bytes", ctypes.c_size_t),
+                ("spec_bytes", ctypes.c_size_t), ("ctx_bytes", ctypes.c_size_t)]
+
+
+_c = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int)
+
+# every symbol declared in include/spm_iqc.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "spm_create": (_i, [ctypes.POINTER(_Params), ctypes.POINTER(_c)]),
+    "spm_destroy": (_i, [_c]),
+    "spm_set_stream": (_i, [_c, _c]),
+    "spm_get_layout": (_i, [_c, ctypes.POINTER(_Layout)]),
+    "spm_last_error": (ctypes.c_char_p, []),
+    "spm_get_GK": (_d, [_d]),
+    "spm_my_fft": (_i, [_c, _c, _c]),
+    "spm_my_ifft": (_i, [_c, _c, _c]),
+    "spm_r2c": (_i, [_c, _c, _c]),
+    "spm_c2r": (_i, [_c, _c, _c]),
+    "spm_energy": (_i, [_c, _c, _pd]),
+    "spm_ngrad": (_i, [_c, _c, _c, _pd]),
+    "spm_gradflow": (_i, [_c, _c, _i, _d, _d, _i, _c, _pi]),
+    "spm_gf_begin": (_i, [_c, _c, _i, _d, _d]),
+    "spm_gf_steps": (_i, [_c, _i]),
+    "spm_gf_end": (_i, [_c, _c, _c, _pi]),
+    "spm_hv": (_i, [_c, _c, _c, _c]),
+    "spm_precond": (_i, [_c, _c]),
+    "spm_inpfp": (_i, [_c, _c, _c, _pd]),
+    "spm_nrmp": (_i, [_c, _c, _pd]),
+    "spm_lin2": (_i, [_c, _c, _d, _c, _d, _c]),
+    "spm_cplx_to_real": (_i, [_c, _c, _c]),
+    "spm_real_to_cplx": (_i, [_c, _c, _c]),
+}
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SpmError("libspm_iqc.so is not built: run `python spm-iqc_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise SpmError("%s failed (%d): %s" % (what, rc, lib().spm_last_error().decode()))
+
+
+def get_GK(k2: float) -> float:
+    """Initialization.cpp:150-166 (host)."""
+    return lib().spm_get_GK(float(k2))
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise SpmError("no GPU: the spm-iqc_amd path runs only on the device")
+    return torch
+
+
+class Grid:
+    """One device context: an N^3 grid, the model parameters and a precision.
+
+    Fields are torch tensors on the GPU: real fields of shape (N, N, N) in the
+    context dtype (float64 / float32), half spectra of shape (N, N, nhp)
+    complex.  Mirrors the reference's process-global state (Data.h:4-28).
+    """
+
+    def __init__(self, N, precision=FP64, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3):
+        torch = _torch()
+        self.torch = torch
+        p = _Params(int(N), int(precision), float(L), float(kscale), float(Tau), float(Gamma), float(dt))
+        h = _c()
+        _check(lib().spm_create(ctypes.byref(p), ctypes.byref(h)), "spm_create")
+        self.h = h
+        lay = _Layout()
+        _check(lib().spm_get_layout(self.h, ctypes.byref(lay)), "spm_get_layout")
+        self.N, self.nh, self.nhp = lay.N, lay.nh, lay.nhp
+        self.R, self.C = lay.R, lay.C
+        self.ctx_bytes = lay.ctx_bytes
+        self.precision = int(precision)
+        self.dtype = torch.float64 if precision == FP64 else torch.float32
+        self.cdtype = torch.complex128 if precision == FP64 else torch.complex64
+        self.params = dict(L=L, kscale=kscale, Tau=Tau, Gamma=Gamma, dt=dt)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- helpers ------------------------------------------------------------
+    def _stream(self):
+        _check(lib().spm_set_stream(self.h, _c(self.torch.cuda.current_stream().cuda_stream)), "spm_set_stream")
+
+    def _ptr(self, t, dtype, shape=None):
+        if not t.is_cuda:
+            raise SpmError("field must be a GPU tensor")
+        if t.dtype != dtype:
+            raise SpmError("field dtype %s, expected %s" % (t.dtype, dtype))
+        if not t.is_contiguous():
+            raise SpmError("field must be contiguous")
+        if shape is not None and t.numel() != int(np.prod(shape)):
+            raise SpmError("field has %d elements, expected %s" % (t.numel(), shape))
+        return _c(t.data_ptr())
+
+    def _real(self, t):
+        return self._ptr(t, self.dtype, (self.N,) * 3)
+
+    def _spec(self, t):
+        return self._ptr(t, self.cdtype, (self.N, self.N, self.nhp))
+
+    def _full(self, t):
+        return self._ptr(t, self.cdtype, (self.N,) * 3)
+
+    def real_field(self, values=None):
+        t = self.torch.zeros((self.N,) * 3, dtype=self.dtype, device="cuda")
+        if values is not None:
+            t.copy_(self.torch.as_tensor(np.asarray(values)).reshape(t.shape))
+        return t
+
+    def spectrum(self):
+        return self.torch.zeros((self.N, self.N, self.nhp), dtype=self.cdtype, device="cuda")
+
+    # -- My_FFT -------------------------------------------------------------
+    def my_fft(self, x_R, x_F=None):
+        """My_FFT.cpp:4-14: c2c forward / N^3 on a full complex (N,N,N) field."""
+        self._stream()
+        x_F = self.torch.empty_like(x_R) if x_F is None else x_F
+        _check(lib().spm_my_fft(self.h, self._full(x_R), self._full(x_F)), "spm_my_fft")
+        return x_F
+
+    def my_ifft(self, x_F, x_R=None):
+        """My_FFT.cpp:16-21: c2c inverse, unnormalised."""
+        self._stream()
+        x_R = self.torch.empty_like(x_F) if x_R is None else x_R
+        _check(lib().spm_my_ifft(self.h, self._full(x_F), self._full(x_R)), "spm_my_ifft")
+        return x_R
+
+    def r2c(self, x, S=None):
+        self._stream()
+        S = self.spectrum() if S is None else S
+        _check(lib().spm_r2c(self.h, self._real(x), self._spec(S)), "spm_r2c")
+        return S
+
+    def c2r(self, S, x=None):
+        self._stream()
+        x = self.real_field() if x is None else x
+        _check(lib().spm_c2r(self.h, self._spec(S), self._real(x)), "spm_c2r")
+        return x
+
+    # -- LPsrc --------------------------------------------------------------
+    def energy(self, phi) -> float:
+        """get_Ene (LPsrc.cpp:8-38) of the real field phi."""
+        self._stream()
+        E = ctypes.c_double()
+        _check(lib().spm_energy(self.h, self._real(phi), ctypes.byref(E)), "spm_energy")
+        return E.value
+
+    def ngrad(self, phi, grad=None, want_norm=True):
+        """ngrad_cam (LPsrc.cpp:119-173); returns (grad, nrmp(grad))."""
+        self._stream()
+        grad = self.real_field() if grad is None else grad
+        nrm = ctypes.c_double()
+        _check(lib().spm_ngrad(self.h, self._real(phi), self._real(grad),
+                               ctypes.byref(nrm) if want_norm else None), "spm_ngrad")
+        return grad, (nrm.value if want_norm else None)
+
+    def gradflow(self, phi, max_iter, res_tol=1e-16, gn_tol=0.0, check_every=64):
+        """get_min_u_F (LPsrc.cpp:40-117) on phi in place.
+
+        Returns (iters, hist) with hist[n] = (E, ||grad F||, res) of iteration n.
+        """
+        self._stream()
+        hist = np.zeros((max(int(max_iter), 1), 3))
+        it = ctypes.c_int()
+        _check(lib().spm_gradflow(self.h, self._real(phi), int(max_iter), float(res_tol), float(gn_tol),
+                                  int(check_every), hist.ctypes.data_as(_c), ctypes.byref(it)), "spm_gradflow")
+        return it.value, hist[: it.value]
+
+    def gf_begin(self, phi, max_iter, res_tol=1e-16, gn_tol=0.0):
+        self._stream()
+        _check(lib().spm_gf_begin(self.h, self._real(phi), int(max_iter), float(res_tol), float(gn_tol)),
+               "spm_gf_begin")
+
+    def gf_steps(self, n):
+        """Enqueue n gradient-flow iterations (asynchronous)."""
+        _check(lib().spm_gf_steps(self.h, int(n)), "spm_gf_steps")
+
+    def gf_end(self, phi=None, max_iter=0):
+        self._stream()
+        hist = np.zeros((max(int(max_iter), 1), 3))
+        it = ctypes.c_int()
+        _check(lib().spm_gf_end(self.h, self._real(phi) if phi is not None else None,
+                                hist.ctypes.data_as(_c), ctypes.byref(it)), "spm_gf_end")
+        return it.value, hist[: min(it.value, len(hist))]
+
+    # -- LOBPCG operators ----------------------------------------------------
+    def hv(self, phi, v, hv=None):
+        """hv_cam (LOBPCG.cpp:113-166)."""
+        self._stream()
+        hv = self.real_field() if hv is None else hv
+        _check(lib().spm_hv(self.h, self._real(phi), self._real(v), self._real(hv)), "spm_hv")
+        return hv
+
+    def precond(self, w):
+        """precond_camnew (LOBPCG.cpp:168-199), in place."""
+        self._stream()
+        _check(lib().spm_precond(self.h, self._real(w)), "spm_precond")
+        return w
+
+    # -- BasicOperators -------------------------------------------------------
+    def inpfp(self, p, q) -> float:
+        self._stream()
+        out = ctypes.c_double()
+        _check(lib().spm_inpfp(self.h, self._real(p), self._real(q), ctypes.byref(out)), "spm_inpfp")
+        return out.value
+
+    def nrmp(self, p) -> float:
+        self._stream()
+        out = ctypes.c_double()
+        _check(lib().spm_nrmp(self.h, self._real(p), ctypes.byref(out)), "spm_nrmp")
+        return out.value
+
+    def lin2(self, r, a1, F1, a2, F2):
+        self._stream()
+        _check(lib().spm_lin2(self.h, self._real(r), float(a1), self._real(F1), float(a2), self._real(F2)),
+               "spm_lin2")
+        return r

@@ -1,0 +1,160 @@
+// grid_impl.h -- GridIface for one (precision, N): the pass sequences of the
+// transforms and operators of the reference's path, launched on ctx.stream.
+#pragma once
+#include "spm_ctx.h"
+
+namespace spm {
+
+template <typename T, int N> struct GridImpl final : GridIface {
+    using G = Geo<T, N>;
+    using CT = C2<T>;
+    static constexpr long NPAIRS = (long)N * N / 2; // row pairs of one field
+
+    void geometry(long &nb_rows, long &nb_j, long &nb_i, int NHP) override
+    {
+        nb_rows = (NPAIRS + G::PP - 1) / G::PP;
+        nb_j = (long)N * ((NHP + G::B - 1) / G::B);
+        nb_i = ((long)N * NHP + G::B - 1) / G::B;
+    }
+
+    // ---------------------------------------------------------- launchers
+    template <bool IN_SPEC, bool OUT_SPEC, class Op>
+    static void rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl = nullptr)
+    {
+        const long nb = (NPAIRS + G::PP - 1) / G::PP;
+        hipLaunchKernelGGL((k_rows<T, N, IN_SPEC, OUT_SPEC, Op>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0,
+                           c.stream, in, (CT *)out, c.NHP, NPAIRS, (const CT *)c.tw, op, part, ctl);
+    }
+    // pencils along j inside each i plane; row length nhp, valid nh
+    template <bool FWD, bool INV, bool STORE, class SOp>
+    static void cols_j(Ctx &c, const void *src, void *dst, SOp op, double *part, const Ctl *ctl = nullptr,
+                       int nh = -1, int nhp = -1)
+    {
+        if (nh < 0) { nh = c.NH; nhp = c.NHP; }
+        const long nb = (long)N * ((nhp + G::B - 1) / G::B);
+        hipLaunchKernelGGL((k_cols<T, N, FWD, INV, STORE, SOp>), dim3((unsigned)nb), dim3(G::COL_THREADS), 0,
+                           c.stream, (const CT *)src, (CT *)dst, (long)nhp, (long)nhp, (long)N * nhp, nh, nhp, 0,
+                           (const CT *)c.tw, op, part, ctl);
+    }
+    // pencils along i over the whole array
+    template <bool FWD, bool INV, bool STORE, class SOp>
+    static void cols_i(Ctx &c, const void *src, void *dst, SOp op, double *part, const Ctl *ctl = nullptr,
+                       int nh = -1, int nhp = -1)
+    {
+        if (nh < 0) { nh = c.NH; nhp = c.NHP; }
+        const long ncols = (long)N * nhp;
+        const long nb = (ncols + G::B - 1) / G::B;
+        hipLaunchKernelGGL((k_cols<T, N, FWD, INV, STORE, SOp>), dim3((unsigned)nb), dim3(G::COL_THREADS), 0,
+                           c.stream, (const CT *)src, (CT *)dst, ncols, ncols, 0L, nh, nhp, 1, (const CT *)c.tw, op,
+                           part, ctl);
+    }
+
+    // ---------------------------------------------------------- transforms
+    void r2c(Ctx &c, const void *x, void *S) override
+    {
+        rows<false, true>(c, x, S, OpNone{}, nullptr);
+        cols_j<true, false, true>(c, S, S, SOpNone{}, nullptr);
+        cols_i<true, false, true>(c, S, S, SOpScale<T>{1.0 / (double)c.R}, nullptr);
+    }
+
+    void c2r(Ctx &c, const void *S, void *x) override
+    {
+        cols_i<false, true, true>(c, S, c.W, SOpNone{}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+    }
+
+    void c2c(Ctx &c, const void *in, void *out, int dir) override
+    {
+        const long nrows = (long)N * N;
+        const long nb = (nrows + G::PP - 1) / G::PP;
+        if (dir < 0) {
+            hipLaunchKernelGGL((k_rows_c2c<T, N, -1>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0, c.stream,
+                               (const CT *)in, (CT *)out, nrows, 1.0, (const CT *)c.tw);
+            cols_j<true, false, true>(c, out, out, SOpNone{}, nullptr, nullptr, N, N);
+            cols_i<true, false, true>(c, out, out, SOpScale<T>{1.0 / (double)c.R}, nullptr, nullptr, N, N);
+        } else {
+            hipLaunchKernelGGL((k_rows_c2c<T, N, +1>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0, c.stream,
+                               (const CT *)in, (CT *)out, nrows, 1.0, (const CT *)c.tw);
+            cols_j<false, true, true>(c, out, out, SOpNone{}, nullptr, nullptr, N, N);
+            cols_i<false, true, true>(c, out, out, SOpNone{}, nullptr, nullptr, N, N);
+        }
+    }
+
+    // ---------------------------------------------------------- operators
+    // get_Ene (LPsrc.cpp:8-38) of a real field, without its two FFTs: the
+    // local term is the mean of the density, the non-local term the weighted
+    // spectral sum of G |phi_hat|^2.
+    void energy(Ctx &c, const void *phi) override
+    {
+        rows<false, true>(c, phi, c.W, OpDensity<T>{c.Tau, c.Gamma}, c.slot(0));
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, false, false>(c, c.W, c.W, SOpEnergy<T, N>{1.0 / (double)c.R, c.Gtab}, c.slot(1));
+        launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_E2);
+        launch_reduce(c, c.slot(1), c.nb_i, 1, 0u, 1.0, c.scal + SC_E1);
+    }
+
+    // ngrad_cam (LPsrc.cpp:119-173)
+    void ngrad(Ctx &c, const void *phi, void *grad) override
+    {
+        rows<false, true>(c, phi, c.W, OpNgradIn<T>{c.Tau, c.Gamma}, c.slot(0));
+        launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_F);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr,
+                          OpNgradOut<T>{c.Tau, c.Gamma, (const T *)phi, c.scal + SC_MEAN_F, (T *)grad}, c.slot(1));
+        launch_reduce(c, c.slot(1), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_GN2);
+    }
+
+    // hv_cam (LOBPCG.cpp:113-166)
+    void hv(Ctx &c, const void *phi, const void *v, void *hvo) override
+    {
+        rows<false, true>(c, v, c.W, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0));
+        launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr,
+                          OpHvOut<T>{c.Tau, c.Gamma, (const T *)phi, (const T *)v, c.scal + SC_MEAN_CV, (T *)hvo},
+                          nullptr);
+    }
+
+    // precond_camnew (LOBPCG.cpp:168-199): w <- IFFT(FFT(w)/N^3 / (G+|Tau|+0.02))
+    // minus its mean (= the same with the DC set to zero).
+    void precond(Ctx &c, void *w) override
+    {
+        rows<false, true>(c, w, c.W, OpNone{}, nullptr);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)w}, nullptr);
+    }
+
+    // ---------------------------------------------------------- gradient flow
+    // State: U = u_F (normalised spectrum of phi).  W carries the spectrum of
+    // f(phi) between passes.  Slot 2 holds the E2 sums of the current phi
+    // (row pass), slot 3 the E1 / ||grad F||^2 / residual sums (i pass).
+    void gf_init(Ctx &c, const void *phi) override
+    {
+        r2c(c, phi, c.U);
+        rows<false, true>(c, phi, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr}, c.slot(2), c.ctl);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
+    }
+
+    // One get_min_u_F iteration (LPsrc.cpp:52-111).  Kernels return at once
+    // when ctl->stop is set, so a fixed sequence of steps can be queued (or
+    // replayed from a graph) past convergence.
+    void gf_step(Ctx &c) override
+    {
+        cols_i<true, true, true>(c, c.W, c.W,
+                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, c.Gtab, (CT *)c.U}, c.slot(3),
+                                 c.ctl);
+        launch_gf_finalize(c, c.slot(3), c.nb_i, c.slot(2), c.nb_rows);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
+        rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr}, c.slot(2), c.ctl);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
+    }
+};
+
+} // namespace spm

@@ -1,0 +1,429 @@
+// grid_ops.h -- 3-D FFT passes over one N^3 grid and the pointwise work of
+// the Landau free-energy path fused into them (gfx950, HIP).
+//
+// Storage on the device (x-major, matching the reference's index order
+// idx = (i*N + j)*N + k, Initialization.cpp:109-120):
+//   real field  T[N][N][N]
+//   half spectrum  C2<T>[N][N][NHP], kz = 0..N/2 valid, NHP >= N/2+1 padded so
+//   every row starts on a 128-byte boundary.  Entry (i, j, kz) holds the
+//   reference's u_F_local[(i*N + j)*N + kz] (normalised by 1/N^3); the other
+//   half follows from Hermitian symmetry.
+//
+// A 3-D transform is three passes, one per axis, each reading and writing
+// the array once:
+//   k axis  "row" pass: two real rows packed as one complex row, one length-N
+//           complex FFT, separated into two half rows (and the reverse).
+//   j axis  column pass over each i plane (pencils of stride NHP).
+//   i axis  column pass over the whole array (pencils of stride N*NHP); the
+//           pointwise spectral operator (pair kernel G(k), preconditioner,
+//           semi-implicit update, energy sums) runs here between the forward
+//           and the inverse transform, so the spectrum of a step never makes
+//           an extra trip through HBM.
+#pragma once
+#include "fft_core.h"
+
+namespace spm {
+
+// Device control block for iterative loops (captured in graphs: kernels read
+// it instead of taking a step index as an argument).
+struct Ctl {
+    int iter;       // iterations completed
+    int stop;       // nonzero: later step kernels return immediately
+    int max_iter;
+    int flags;
+    double res_tol; // stop when res <= res_tol
+    double gn_tol;  // stop when ||grad F|| <= gn_tol
+};
+
+template <typename T, int N> struct Geo {
+    static constexpr int TPP = FftPlan<N>::TPP, E = FftPlan<N>::E;
+    // column passes: B adjacent columns per block -> 128-byte row segments
+    static constexpr int B0 = 128 / (int)sizeof(C2<T>);
+    static constexpr int B = (64 / TPP > B0) ? 64 / TPP : B0;
+    static constexpr int COL_THREADS = B * TPP;
+    // row passes: PP row pairs per block
+    static constexpr int PP0 = (sizeof(T) == 8 ? 128 : 256) / TPP;
+    static constexpr int PP = PP0 > 0 ? PP0 : 1;
+    static constexpr int ROW_THREADS = PP * TPP;
+    static constexpr int RSTR = N + 1; // LDS stride between rows (complex)
+};
+
+__device__ __forceinline__ int kfold(int i, int N) { return i < N / 2 ? i : i - N; }
+
+// ======================================================================
+// Real-space operators applied to each real value inside the row pass.
+// op(x, g, acc): x = value at real-field index g; returns the value the pass
+// transforms next (if it does), may write outputs and accumulate FP64 sums.
+// ======================================================================
+struct OpNone {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    template <typename T> __device__ __forceinline__ T operator()(T x, long, double *) const { return x; }
+};
+
+template <typename T> struct OpStore { // c2r output: out[g] = x
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    T *out;
+    __device__ __forceinline__ T operator()(T x, long g, double *) const
+    {
+        out[g] = x;
+        return x;
+    }
+};
+
+// Local free-energy density and nonlinear force of get_Ene / get_min_u_F
+// (LPsrc.cpp:11-21, 56-61).  Sums the density (E2 = its mean) and returns
+// f = Tau*u + Gamma*u^2 - u^3.  Optionally stores u.
+template <typename T> struct OpGradFlowRow {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    T *phi_out;
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    {
+        const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+        acc[0] += -0.5 * Tau * u2 - Gamma * u3 / 3 + 0.25 * u4;
+        if (phi_out) phi_out[g] = x;
+        return (T)(Tau * u + Gamma * u2 - u3);
+    }
+};
+
+// Energy density only (get_Ene's nln, LPsrc.cpp:11-21); passes u through.
+template <typename T> struct OpDensity {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    __device__ __forceinline__ T operator()(T x, long, double *acc) const
+    {
+        const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+        acc[0] += -0.5 * Tau * u2 - Gamma * u3 / 3 + 0.25 * u4;
+        return x;
+    }
+};
+
+// ngrad_cam input pass: transform phi, sum f(phi) (its mean is the mean of
+// the gradient before it is removed, LPsrc.cpp:147-167).
+template <typename T> struct OpNgradIn {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    __device__ __forceinline__ T operator()(T x, long, double *acc) const
+    {
+        const double u = (double)x, u2 = u * u;
+        acc[0] += Tau * u + Gamma * u2 - u2 * u;
+        return x;
+    }
+};
+
+// ngrad_cam epilogue (LPsrc.cpp:147-169): x = ifft(G phi_hat, DC = 0);
+// grad = -((x - Tau u - Gamma u^2 + u^3) - mean), mean = -mean(f).
+// Sums grad^2 for nrmp.
+template <typename T> struct OpNgradOut {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    const T *phi;
+    const double *mean_f; // device scalar: mean of f(phi)
+    T *grad;
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    {
+        const double u = (double)phi[g], u2 = u * u, u3 = u2 * u;
+        const double gr = (double)x - Tau * u - Gamma * u2 + u3;
+        const double out = -(gr + *mean_f);
+        grad[g] = (T)out;
+        acc[0] += out * out;
+        return x;
+    }
+};
+
+// hv_cam input pass: transform v, sum c(phi) v with c = -Tau - 2 Gamma phi + 3 phi^2.
+template <typename T> struct OpHvIn {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    const T *phi;
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    {
+        const double u = (double)phi[g];
+        acc[0] += (-Tau - 2 * Gamma * u + 3 * u * u) * (double)x;
+        return x;
+    }
+};
+
+// hv_cam epilogue (LOBPCG.cpp:136-156): hv = x + c(phi) v - mean(c v), with x
+// = ifft(G v_hat, DC = 0) (the DC of G v_hat is the mean of ifft(G v_hat)).
+template <typename T> struct OpHvOut {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    const T *phi, *v;
+    const double *mean_cv;
+    T *hv;
+    __device__ __forceinline__ T operator()(T x, long g, double *) const
+    {
+        const double u = (double)phi[g];
+        hv[g] = (T)((double)x + (-Tau - 2 * Gamma * u + 3 * u * u) * (double)v[g] - *mean_cv);
+        return x;
+    }
+};
+
+// ======================================================================
+// Spectral operators applied inside the i-axis column pass.
+// op(v, s, i, j, kz, acc): v = (unnormalised forward) value at spectrum index
+// s with frequency indices (i, j, kz); may modify v, touch other arrays at s.
+// ======================================================================
+struct SOpNone {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    template <typename T> __device__ __forceinline__ void operator()(C2<T> &, long, int, int, int, double *) const {}
+};
+
+template <typename T> struct SOpScale { // r2c completion: v *= sc
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    double sc;
+    __device__ __forceinline__ void operator()(C2<T> &v, long, int, int, int, double *) const
+    {
+        v.x = (T)((double)v.x * sc);
+        v.y = (T)((double)v.y * sc);
+    }
+};
+
+// v <- sc * tab[|n|^2] * v, DC set to zero.  tab = G (ngrad_cam, hv_cam) or
+// 1/(G + |Tau| + 0.02) (precond_camnew).
+template <typename T, int N> struct SOpMulTab {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    double sc;
+    const double *tab;
+    __device__ __forceinline__ void operator()(C2<T> &v, long, int i, int j, int kz, double *) const
+    {
+        const int a = kfold(i, N), b = kfold(j, N);
+        const int n2 = a * a + b * b + kz * kz;
+        const double f = n2 == 0 ? 0.0 : sc * tab[n2];
+        v.x = (T)((double)v.x * f);
+        v.y = (T)((double)v.y * f);
+    }
+};
+
+// Hermitian weight of a half-spectrum entry in a sum over the full spectrum.
+template <int N> __device__ __forceinline__ double hweight(int kz) { return (kz == 0 || kz == N / 2) ? 1.0 : 2.0; }
+
+// Non-local energy sum of get_Ene (LPsrc.cpp:27-35): acc += w G |sc v|^2.
+template <typename T, int N> struct SOpEnergy {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double sc;
+    const double *G;
+    __device__ __forceinline__ void operator()(C2<T> &v, long, int i, int j, int kz, double *acc) const
+    {
+        const int a = kfold(i, N), b = kfold(j, N);
+        const double re = (double)v.x * sc, im = (double)v.y * sc;
+        acc[0] += hweight<N>(kz) * G[a * a + b * b + kz * kz] * (re * re + im * im);
+    }
+};
+
+// One semi-implicit gradient-flow update (LPsrc.cpp:63-84) on the state
+// spectrum U: f_hat = sc * v; U' = (dt f_hat + U) / (1 + dt G), DC = 0.
+// Sums, for the pre-step state: w G |U|^2 (2 E1), w |G U - f_hat|^2 over
+// k != 0 (||grad F||^2 by Parseval), and max |Re(U' - U)| (the reference's
+// residual).  v becomes U' (then inverse-transformed by the same pass).
+template <typename T, int N> struct SOpGradFlowStep {
+    static constexpr int NRED = 3;
+    static constexpr unsigned MAXM = 4u; // acc[2] is a max
+    double sc, dt;
+    const double *G;
+    C2<T> *U;
+    __device__ __forceinline__ void operator()(C2<T> &v, long s, int i, int j, int kz, double *acc) const
+    {
+        const int a = kfold(i, N), b = kfold(j, N);
+        const int n2 = a * a + b * b + kz * kz;
+        const double g = G[n2], w = hweight<N>(kz);
+        const C2<T> u = U[s];
+        const double fr = (double)v.x * sc, fi = (double)v.y * sc;
+        const double ur = (double)u.x, ui = (double)u.y;
+        double nr = (dt * fr + ur) / (1 + dt * g), ni = (dt * fi + ui) / (1 + dt * g);
+        if (n2 == 0) {
+            nr = 0.0;
+            ni = 0.0;
+        } else {
+            const double dr = g * ur - fr, di = g * ui - fi;
+            acc[1] += w * (dr * dr + di * di);
+        }
+        acc[0] += w * g * (ur * ur + ui * ui);
+        const C2<T> un = {(T)nr, (T)ni};
+        acc[2] = fmax(acc[2], fabs((double)un.x - ur));
+        U[s] = un;
+        v = un;
+    }
+};
+
+// ======================================================================
+// Kernels
+// ======================================================================
+
+template <int NRED, unsigned MAXM>
+__device__ __forceinline__ void write_partials(double (&acc)[NRED > 0 ? NRED : 1], double *red, double *part)
+{
+    if constexpr (NRED > 0) {
+        block_reduce<NRED, MAXM>(acc, red);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < NRED; ++q) part[(long)q * gridDim.x + blockIdx.x] = acc[q];
+    }
+}
+
+// Row (k-axis) pass over npairs pairs of rows.
+//   IN_SPEC:  input rows are half spectra (row stride NHP): c2r (unnormalised)
+//   !IN_SPEC: input rows are real (row stride N)
+//   OUT_SPEC: forward r2c (unnormalised) of op(x) into out (row stride NHP)
+template <typename T, int N, bool IN_SPEC, bool OUT_SPEC, class Op>
+__global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
+    k_rows(const void *in, C2<T> *out, int NHP, long npairs,
+           const C2<T> *__restrict__ tw, Op op, double *__restrict__ part, const Ctl *__restrict__ ctl)
+{
+    using G = Geo<T, N>;
+    constexpr int TPP = G::TPP, E = G::E, PP = G::PP;
+    __shared__ C2<T> lds[PP * G::RSTR];
+    __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16];
+    if (ctl && ctl->stop) return;
+    const int t = threadIdx.x % TPP, pp = threadIdx.x / TPP;
+    const long pair = (long)blockIdx.x * PP + pp;
+    const bool active = pair < npairs;
+    const long ra = 2 * pair, rb = ra + 1;
+    C2<T> *my = lds + pp * G::RSTR;
+    C2<T> v[E];
+    double acc[Op::NRED > 0 ? Op::NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (Op::NRED > 0 ? Op::NRED : 1); ++q) acc[q] = 0.0;
+
+    if constexpr (IN_SPEC) {
+        const C2<T> *S = (const C2<T> *)in;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPP * m;
+            const int q = p <= N / 2 ? p : N - p;
+            C2<T> a = {0, 0}, b = {0, 0};
+            if (active) {
+                a = S[ra * NHP + q];
+                b = S[rb * NHP + q];
+            }
+            if (q == 0 || q == N / 2) { // c2r reads only the real part of DC and Nyquist
+                a.y = 0;
+                b.y = 0;
+            }
+            // Z = Xa + i Xb on 0..N/2, conj(Xa) + i conj(Xb) above
+            v[m] = p <= N / 2 ? C2<T>{a.x - b.y, a.y + b.x} : C2<T>{a.x + b.y, b.x - a.y};
+        }
+        fft_pencil<T, N, +1, 1>(v, my, tw, t);
+    } else {
+        const T *X = (const T *)in;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPP * m;
+            v[m] = active ? C2<T>{X[ra * N + p], X[rb * N + p]} : C2<T>{0, 0};
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPP * m;
+            v[m].x = op(v[m].x, ra * N + p, acc);
+            v[m].y = op(v[m].y, rb * N + p, acc);
+        }
+    }
+    if constexpr (OUT_SPEC) {
+        fft_pencil<T, N, -1, 1>(v, my, tw, t);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m];
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int m = 0; m <= E / 2; ++m) {
+                const int k = t + TPP * m;
+                if (k <= N / 2) {
+                    const C2<T> A = my[k], Bc = my[(N - k) % N];
+                    // B = conj(Z[N-k]); Xa = (A + B)/2, Xb = -i (A - B)/2
+                    const T hx = (T)0.5;
+                    out[ra * NHP + k] = C2<T>{hx * (A.x + Bc.x), hx * (A.y - Bc.y)};
+                    out[rb * NHP + k] = C2<T>{hx * (A.y + Bc.y), hx * (Bc.x - A.x)};
+                }
+            }
+        }
+    }
+    write_partials<Op::NRED, Op::MAXM>(acc, red, part);
+}
+
+// Complex rows (the reference's full c2c layout): length-N transforms of
+// nrows contiguous rows, out = sc * DFT_DIR(in).
+template <typename T, int N, int DIR>
+__global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
+    k_rows_c2c(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, long nrows, double sc,
+               const C2<T> *__restrict__ tw)
+{
+    using G = Geo<T, N>;
+    constexpr int TPP = G::TPP, E = G::E, PP = G::PP;
+    __shared__ C2<T> lds[PP * G::RSTR];
+    const int t = threadIdx.x % TPP, pp = threadIdx.x / TPP;
+    const long row = (long)blockIdx.x * PP + pp;
+    const bool active = row < nrows;
+    C2<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = active ? in[row * N + t + TPP * m] : C2<T>{0, 0};
+    fft_pencil<T, N, DIR, 1>(v, lds + pp * G::RSTR, tw, t);
+    if (active)
+#pragma unroll
+        for (int m = 0; m < E; ++m) out[row * N + t + TPP * m] = cscale(v[m], (T)sc);
+}
+
+// Column pass, pencils of length N along the j or i axis.
+// Element e of column col of outer block o: src[o*ostride + e*pstride + col].
+// Valid columns: col < ncols and (col mod NHP) < NH.  iaxis: the pencil runs
+// along i (frequencies i = e, j = col / NHP, kz = col mod NHP); otherwise
+// along j (i = o, j = e, kz = col).
+template <typename T, int N, bool FWD, bool INV, bool STORE, class SOp>
+__global__ void __launch_bounds__((Geo<T, N>::COL_THREADS))
+    k_cols(const C2<T> *src, C2<T> *dst, long pstride, long ncols, long ostride, int NH, int NHP,
+           int iaxis, const C2<T> *__restrict__ tw, SOp op, double *__restrict__ part,
+           const Ctl *__restrict__ ctl)
+{
+    using G = Geo<T, N>;
+    constexpr int TPP = G::TPP, E = G::E, B = G::B;
+    __shared__ C2<T> lds[N * B];
+    __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16];
+    if (ctl && ctl->stop) return;
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const long tpo = (ncols + B - 1) / B;
+    const long o = blockIdx.x / tpo;
+    const long col = (blockIdx.x % tpo) * B + c;
+    const int kz = (int)(col % NHP);
+    const bool active = col < ncols && kz < NH;
+    const long off = o * ostride + col;
+    C2<T> v[E];
+    double acc[SOp::NRED > 0 ? SOp::NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (SOp::NRED > 0 ? SOp::NRED : 1); ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = active ? src[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
+    if constexpr (FWD) fft_pencil<T, N, -1, B>(v, lds + c, tw, t);
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int e = t + TPP * m;
+            const int i = iaxis ? e : (int)o;
+            const int j = iaxis ? (int)(col / NHP) : e;
+            op(v[m], off + (long)e * pstride, i, j, kz, acc);
+        }
+    }
+    if constexpr (INV) fft_pencil<T, N, +1, B>(v, lds + c, tw, t);
+    if constexpr (STORE) {
+        if (active)
+#pragma unroll
+            for (int m = 0; m < E; ++m) dst[off + (long)(t + TPP * m) * pstride] = v[m];
+    }
+    write_partials<SOp::NRED, SOp::MAXM>(acc, red, part);
+}
+
+} // namespace spm

@@ -1,0 +1,595 @@
+// spm_capi.hip -- the C ABI (include/spm_iqc.h): context set-up, tables,
+// generic vector kernels and reductions, and the host side of each entry
+// point of the reference's path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spm_iqc.h"
+#include "spm_ctx.h"
+
+using namespace spm;
+
+struct spm_ctx {
+    Ctx c;
+    hipStream_t own = nullptr;
+    double *h_pin = nullptr; // pinned host scratch for scalars read back
+    size_t dev_bytes = 0;
+};
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail(SPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+// ======================================================================
+// generic kernels
+// ======================================================================
+
+// out[q] = scale * sum_b part[q*nblk + b] (or max over b if bit q of maxmask)
+__global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part, long nblk, unsigned maxmask,
+                                               double scale, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    const int q = blockIdx.x;
+    const bool mx = (maxmask >> q) & 1u;
+    double s = mx ? 0.0 : 0.0;
+    for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double x = part[(long)q * nblk + b];
+        s = mx ? fmax(s, x) : s + x;
+    }
+    double v[1] = {s};
+    if (mx) block_reduce<1, 1u>(v, red);
+    else block_reduce<1, 0u>(v, red);
+    if (threadIdx.x == 0) out[q] = mx ? v[0] : v[0] * scale;
+}
+
+// Closes one gradient-flow iteration: E = E1 + E2, ||grad F||, residual into
+// hist[3*iter ..], then iter += 1 and the stop test of LPsrc.cpp:52.
+__global__ void __launch_bounds__(256) k_gf_finalize(const double *__restrict__ pA, long nA,
+                                                    const double *__restrict__ pC, long nC, double R, Ctl *ctl,
+                                                    double *__restrict__ hist)
+{
+    __shared__ double red[4 * 16];
+    if (ctl->stop) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long b = threadIdx.x; b < nA; b += blockDim.x) {
+        v[0] += pA[b];
+        v[1] += pA[nA + b];
+        v[2] = fmax(v[2], pA[2 * nA + b]);
+    }
+    for (long b = threadIdx.x; b < nC; b += blockDim.x) v[3] += pC[b];
+    block_reduce<4, 4u>(v, red);
+    if (threadIdx.x == 0) {
+        const double E = 0.5 * v[0] + v[3] / R, gn = sqrt(v[1]), res = v[2];
+        const int it = ctl->iter;
+        hist[3L * it + 0] = E;
+        hist[3L * it + 1] = gn;
+        hist[3L * it + 2] = res;
+        ctl->iter = it + 1;
+        if (!(res > ctl->res_tol) || (ctl->gn_tol > 0 && gn <= ctl->gn_tol) || it + 1 >= ctl->max_iter)
+            ctl->stop = 1;
+    }
+}
+
+__global__ void k_ctl_init(Ctl *ctl, int max_iter, double res_tol, double gn_tol)
+{
+    ctl->iter = 0;
+    ctl->stop = max_iter <= 0 ? 1 : 0;
+    ctl->max_iter = max_iter;
+    ctl->flags = 0;
+    ctl->res_tol = res_tol;
+    ctl->gn_tol = gn_tol;
+}
+
+static constexpr int DOT_BLOCKS = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *__restrict__ b, long n,
+                                            double *__restrict__ part)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    double v[1] = {s};
+    block_reduce<1, 0u>(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ void k_lin2(T *r, double a1, const T *F1, double a2, const T *F2, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        r[i] = (T)(a1 * (double)F1[i] + a2 * (double)F2[i]);
+}
+
+template <typename T> __global__ void k_cplx_to_real(const C2<T> *__restrict__ z, T *__restrict__ x, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = z[i].x;
+}
+
+template <typename T> __global__ void k_real_to_cplx(const T *__restrict__ x, C2<T> *__restrict__ z, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        z[i] = C2<T>{x[i], (T)0};
+}
+
+namespace spm {
+
+void launch_reduce(Ctx &c, const double *part, long nblk, int nval, unsigned maxmask, double scale, double *out)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(nval), dim3(256), 0, c.stream, part, nblk, maxmask, scale, out);
+}
+
+void launch_gf_finalize(Ctx &c, const double *pA, long nA, const double *pC, long nC)
+{
+    hipLaunchKernelGGL(k_gf_finalize, dim3(1), dim3(256), 0, c.stream, pA, nA, pC, nC, (double)c.R, c.ctl,
+                       c.hist);
+}
+
+GridIface *make_grid_n4(int), *make_grid_n8(int), *make_grid_n12(int), *make_grid_n16(int),
+    *make_grid_n24(int), *make_grid_n32(int), *make_grid_n48(int), *make_grid_n64(int), *make_grid_n96(int),
+    *make_grid_n128(int), *make_grid_n192(int), *make_grid_n256(int);
+
+GridIface *make_grid(int N, int prec)
+{
+    switch (N) {
+    case 4: return make_grid_n4(prec);
+    case 8: return make_grid_n8(prec);
+    case 12: return make_grid_n12(prec);
+    case 16: return make_grid_n16(prec);
+    case 24: return make_grid_n24(prec);
+    case 32: return make_grid_n32(prec);
+    case 48: return make_grid_n48(prec);
+    case 64: return make_grid_n64(prec);
+    case 96: return make_grid_n96(prec);
+    case 128: return make_grid_n128(prec);
+    case 192: return make_grid_n192(prec);
+    case 256: return make_grid_n256(prec);
+    default: return nullptr;
+    }
+}
+
+} // namespace spm
+
+// ======================================================================
+// host helpers
+// ======================================================================
+
+// Initialization.cpp:150-166
+extern "C" double spm_get_GK(double k2)
+{
+    double ph = 0.568809;
+    double d[5] = {1.969182919080340, -14.949858723012090, 37.445683882581790, -38.839451762550148,
+                   13.999261011773505};
+    double d0 = d[0], d2 = d[1], d4 = d[2], d6 = d[3], d8 = d[4];
+    double s2 = ph * ph;
+    double k4 = k2 * k2, k6 = k4 * k2, k8 = k6 * k2;
+    return exp(-0.5 * k2 / s2) * (d0 + d2 * k2 + d4 * k4 + d6 * k6 + d8 * k8) + 0.08;
+}
+
+static int dev_alloc(spm_ctx *h, void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return fail(SPM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    h->dev_bytes += bytes;
+    return SPM_OK;
+}
+
+static void free_all(spm_ctx *h)
+{
+    Ctx &c = h->c;
+    void *ps[] = {c.tw, c.Gtab, c.Itab, c.W, c.U, c.part, c.scal, c.ctl, c.hist};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (h->h_pin) (void)hipHostFree(h->h_pin);
+    if (h->own) (void)hipStreamDestroy(h->own);
+    delete c.g;
+}
+
+static bool supported_N(int N)
+{
+    switch (N) {
+    case 4: case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
+        return true;
+    default: return false;
+    }
+}
+
+static int ensure_hist(spm_ctx *h, long n)
+{
+    Ctx &c = h->c;
+    if (c.hist_cap >= n) return SPM_OK;
+    if (c.hist) {
+        HIPCHK(hipStreamSynchronize(c.stream));
+        HIPCHK(hipFree(c.hist));
+        h->dev_bytes -= sizeof(double) * 3 * c.hist_cap;
+        c.hist = nullptr;
+    }
+    long cap = 1024;
+    while (cap < n) cap *= 2;
+    int rc = dev_alloc(h, (void **)&c.hist, sizeof(double) * 3 * cap);
+    if (rc) return rc;
+    c.hist_cap = cap;
+    return SPM_OK;
+}
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+const char *spm_last_error(void) { return g_err.c_str(); }
+
+int spm_create(const spm_params *p, spm_ctx **out)
+{
+    if (!p || !out) return fail(SPM_EINVAL, "spm_create: null argument");
+    *out = nullptr;
+    if (!supported_N(p->N)) return fail(SPM_ENOTSUP, "spm_create: unsupported N " + std::to_string(p->N));
+    if (p->precision != SPM_FP64 && p->precision != SPM_FP32)
+        return fail(SPM_EINVAL, "spm_create: precision must be SPM_FP64 or SPM_FP32");
+    if (!(p->L > 0) || !(p->kscale > 0) || !std::isfinite(p->Tau) || !std::isfinite(p->Gamma) || !(p->dt > 0))
+        return fail(SPM_EINVAL, "spm_create: L, kscale, dt must be > 0 and Tau, Gamma finite");
+    spm_ctx *h = new spm_ctx();
+    Ctx &c = h->c;
+    c.N = p->N;
+    c.prec = p->precision;
+    c.esz = c.prec == SPM_FP64 ? 8 : 4;
+    c.NH = c.N / 2 + 1;
+    const int align = c.prec == SPM_FP64 ? 8 : 16; // 128-byte rows
+    c.NHP = (c.NH + align - 1) / align * align;
+    c.R = (long)c.N * c.N * c.N;
+    c.C = (long)c.N * c.N * c.NHP;
+    c.L = p->L;
+    c.kscale = p->kscale;
+    c.Tau = p->Tau;
+    c.Gamma = p->Gamma;
+    c.dt = p->dt;
+    c.g = make_grid(c.N, c.prec);
+    if (!c.g) {
+        delete h;
+        return fail(SPM_ENOTSUP, "spm_create: no kernels for this N");
+    }
+    c.g->geometry(c.nb_rows, c.nb_j, c.nb_i, c.NHP);
+    int rc = SPM_OK;
+    auto bail = [&](int code) {
+        free_all(h);
+        delete h;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SPM_EHIP, "spm_create: hipStreamCreate failed (no GPU?)"));
+    c.stream = h->own;
+    // twiddles exp(-2 pi i e / N)
+    {
+        std::vector<double> twd(2 * c.N);
+        std::vector<float> twf(2 * c.N);
+        for (int e = 0; e < c.N; ++e) {
+            const long double a = 2.0L * 3.14159265358979323846264338327950288L * e / c.N;
+            twd[2 * e] = (double)cosl(a);
+            twd[2 * e + 1] = (double)(-sinl(a));
+            twf[2 * e] = (float)cosl(a);
+            twf[2 * e + 1] = (float)(-sinl(a));
+        }
+        if ((rc = dev_alloc(h, &c.tw, 2 * c.N * c.esz))) return bail(rc);
+        if (hipMemcpy(c.tw, c.prec == SPM_FP64 ? (void *)twd.data() : (void *)twf.data(), 2 * c.N * c.esz,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(SPM_EHIP, "spm_create: twiddle upload failed"));
+    }
+    // G and preconditioner tables over |n|^2 (get_ksquare_local, get_epmckt2_local,
+    // get_ikt2_local; Initialization.cpp:123-174)
+    {
+        const int h2 = c.N / 2;
+        c.ntab = 3 * h2 * h2 + 1;
+        std::vector<double> Gt(c.ntab), It(c.ntab);
+        const double P = c.kscale / c.L;
+        for (int n2 = 0; n2 < c.ntab; ++n2) {
+            const double k2 = (double)((long double)P * (long double)P * (long double)n2);
+            Gt[n2] = spm_get_GK(k2);
+            It[n2] = 1.0 / (Gt[n2] + std::fabs(c.Tau) + 0.02);
+        }
+        if ((rc = dev_alloc(h, (void **)&c.Gtab, sizeof(double) * c.ntab))) return bail(rc);
+        if ((rc = dev_alloc(h, (void **)&c.Itab, sizeof(double) * c.ntab))) return bail(rc);
+        if (hipMemcpy(c.Gtab, Gt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c.Itab, It.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(SPM_EHIP, "spm_create: table upload failed"));
+    }
+    const size_t spec = (size_t)c.C * 2 * c.esz;
+    if ((rc = dev_alloc(h, &c.W, spec))) return bail(rc);
+    if ((rc = dev_alloc(h, &c.U, spec))) return bail(rc);
+    long mb = std::max(std::max(c.nb_rows, c.nb_j), std::max(c.nb_i, (long)DOT_BLOCKS));
+    c.part_slot = 4 * mb;
+    c.nslots = 8;
+    if ((rc = dev_alloc(h, (void **)&c.part, sizeof(double) * c.part_slot * c.nslots))) return bail(rc);
+    if ((rc = dev_alloc(h, (void **)&c.scal, sizeof(double) * SC_COUNT))) return bail(rc);
+    if ((rc = dev_alloc(h, (void **)&c.ctl, sizeof(Ctl)))) return bail(rc);
+    if (hipHostMalloc((void **)&h->h_pin, 4096, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(SPM_ENOMEM, "spm_create: pinned scratch"));
+    if ((rc = ensure_hist(h, 1024))) return bail(rc);
+    if (hipMemset(c.scal, 0, sizeof(double) * SC_COUNT) != hipSuccess)
+        return bail(fail(SPM_EHIP, "spm_create: memset"));
+    *out = h;
+    return SPM_OK;
+}
+
+int spm_destroy(spm_ctx *h)
+{
+    if (!h) return SPM_OK;
+    (void)hipStreamSynchronize(h->c.stream);
+    free_all(h);
+    delete h;
+    return SPM_OK;
+}
+
+int spm_set_stream(spm_ctx *h, void *s)
+{
+    if (!h) return fail(SPM_EINVAL, "null context");
+    h->c.stream = (hipStream_t)s; // NULL: the legacy default stream
+    return SPM_OK;
+}
+
+int spm_get_layout(const spm_ctx *h, spm_layout *o)
+{
+    if (!h || !o) return fail(SPM_EINVAL, "null argument");
+    const Ctx &c = h->c;
+    o->N = c.N;
+    o->nh = c.NH;
+    o->nhp = c.NHP;
+    o->precision = c.prec;
+    o->R = c.R;
+    o->C = c.C;
+    o->real_bytes = (size_t)c.R * c.esz;
+    o->spec_bytes = (size_t)c.C * 2 * c.esz;
+    o->ctx_bytes = h->dev_bytes;
+    return SPM_OK;
+}
+
+#define CHECK_CTX(h)                                                                     \
+    if (!(h)) return fail(SPM_EINVAL, "null context");                                  \
+    Ctx &c = (h)->c;                                                                    \
+    (void)c
+
+static int launch_status(const char *what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SPM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return SPM_OK;
+}
+
+int spm_my_fft(spm_ctx *h, const void *x_R, void *x_F)
+{
+    CHECK_CTX(h);
+    if (!x_R || !x_F) return fail(SPM_EINVAL, "spm_my_fft: null field");
+    c.g->c2c(c, x_R, x_F, -1);
+    return launch_status("spm_my_fft");
+}
+
+int spm_my_ifft(spm_ctx *h, const void *x_F, void *x_R)
+{
+    CHECK_CTX(h);
+    if (!x_R || !x_F) return fail(SPM_EINVAL, "spm_my_ifft: null field");
+    c.g->c2c(c, x_F, x_R, +1);
+    return launch_status("spm_my_ifft");
+}
+
+int spm_r2c(spm_ctx *h, const void *x, void *S)
+{
+    CHECK_CTX(h);
+    if (!x || !S) return fail(SPM_EINVAL, "spm_r2c: null field");
+    c.g->r2c(c, x, S);
+    return launch_status("spm_r2c");
+}
+
+int spm_c2r(spm_ctx *h, const void *S, void *x)
+{
+    CHECK_CTX(h);
+    if (!x || !S) return fail(SPM_EINVAL, "spm_c2r: null field");
+    c.g->c2r(c, S, x);
+    return launch_status("spm_c2r");
+}
+
+static int read_scalars(spm_ctx *h, const double *dev, int n, double *out)
+{
+    Ctx &c = h->c;
+    HIPCHK(hipMemcpyAsync(h->h_pin, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    std::memcpy(out, h->h_pin, sizeof(double) * n);
+    return SPM_OK;
+}
+
+int spm_energy(spm_ctx *h, const void *phi, double *E)
+{
+    CHECK_CTX(h);
+    if (!phi || !E) return fail(SPM_EINVAL, "spm_energy: null argument");
+    c.g->energy(c, phi);
+    int rc = launch_status("spm_energy");
+    if (rc) return rc;
+    double v[2];
+    if ((rc = read_scalars(h, c.scal + SC_E1, 2, v))) return rc;
+    *E = 0.5 * v[0] + v[1]; // E1 + E2 (LPsrc.cpp:35-36)
+    return SPM_OK;
+}
+
+int spm_ngrad(spm_ctx *h, const void *phi, void *grad, double *nrm)
+{
+    CHECK_CTX(h);
+    if (!phi || !grad) return fail(SPM_EINVAL, "spm_ngrad: null field");
+    if (phi == grad) return fail(SPM_EINVAL, "spm_ngrad: phi and grad must differ");
+    c.g->ngrad(c, phi, grad);
+    int rc = launch_status("spm_ngrad");
+    if (rc || !nrm) return rc;
+    double g2;
+    if ((rc = read_scalars(h, c.scal + SC_GN2, 1, &g2))) return rc;
+    *nrm = std::sqrt(g2);
+    return SPM_OK;
+}
+
+int spm_hv(spm_ctx *h, const void *phi, const void *v, void *hv)
+{
+    CHECK_CTX(h);
+    if (!phi || !v || !hv) return fail(SPM_EINVAL, "spm_hv: null field");
+    if (hv == v || hv == phi) return fail(SPM_EINVAL, "spm_hv: hv must not alias phi or v");
+    c.g->hv(c, phi, v, hv);
+    return launch_status("spm_hv");
+}
+
+int spm_precond(spm_ctx *h, void *w)
+{
+    CHECK_CTX(h);
+    if (!w) return fail(SPM_EINVAL, "spm_precond: null field");
+    c.g->precond(c, w);
+    return launch_status("spm_precond");
+}
+
+int spm_gradflow(spm_ctx *h, void *phi, int max_iter, double res_tol, double gn_tol, int check_every,
+                 double *hist, int *iters)
+{
+    CHECK_CTX(h);
+    if (!phi) return fail(SPM_EINVAL, "spm_gradflow: null field");
+    if (max_iter < 0 || check_every < 1) return fail(SPM_EINVAL, "spm_gradflow: max_iter >= 0, check_every >= 1");
+    int rc = ensure_hist(h, max_iter > 0 ? max_iter : 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, c.stream, c.ctl, max_iter, res_tol, gn_tol);
+    if (max_iter > 0) {
+        c.g->gf_init(c, phi);
+        long queued = 0;
+        while (queued < max_iter) {
+            const long n = std::min<long>(check_every, max_iter - queued);
+            for (long s = 0; s < n; ++s) c.g->gf_step(c);
+            queued += n;
+            if ((rc = launch_status("spm_gradflow"))) return rc;
+            HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            if (((Ctl *)h->h_pin)->stop) break;
+        }
+        c.g->c2r(c, c.U, phi);
+    }
+    HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const int it = ((Ctl *)h->h_pin)->iter;
+    if (iters) *iters = it;
+    if (hist && it > 0) {
+        HIPCHK(hipMemcpyAsync(hist, c.hist, sizeof(double) * 3 * it, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+    }
+    return launch_status("spm_gradflow");
+}
+
+int spm_gf_begin(spm_ctx *h, const void *phi, int max_iter, double res_tol, double gn_tol)
+{
+    CHECK_CTX(h);
+    if (!phi) return fail(SPM_EINVAL, "spm_gf_begin: null field");
+    if (max_iter < 0) return fail(SPM_EINVAL, "spm_gf_begin: max_iter >= 0");
+    int rc = ensure_hist(h, max_iter > 0 ? max_iter : 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, c.stream, c.ctl, max_iter, res_tol, gn_tol);
+    if (max_iter > 0) c.g->gf_init(c, phi);
+    return launch_status("spm_gf_begin");
+}
+
+int spm_gf_steps(spm_ctx *h, int nsteps)
+{
+    CHECK_CTX(h);
+    if (nsteps < 0) return fail(SPM_EINVAL, "spm_gf_steps: nsteps >= 0");
+    for (int s = 0; s < nsteps; ++s) c.g->gf_step(c);
+    return launch_status("spm_gf_steps");
+}
+
+int spm_gf_end(spm_ctx *h, void *phi, double *hist, int *iters)
+{
+    CHECK_CTX(h);
+    if (phi) c.g->c2r(c, c.U, phi);
+    HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const int it = ((Ctl *)h->h_pin)->iter;
+    if (iters) *iters = it;
+    if (hist && it > 0) {
+        HIPCHK(hipMemcpyAsync(hist, c.hist, sizeof(double) * 3 * it, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+    }
+    return launch_status("spm_gf_end");
+}
+
+int spm_inpfp(spm_ctx *h, const void *p, const void *q, double *out)
+{
+    CHECK_CTX(h);
+    if (!p || !q || !out) return fail(SPM_EINVAL, "spm_inpfp: null argument");
+    double *part = c.slot(7);
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_dot<double>, dim3(DOT_BLOCKS), dim3(256), 0, c.stream, (const double *)p,
+                           (const double *)q, c.R, part);
+    else
+        hipLaunchKernelGGL(k_dot<float>, dim3(DOT_BLOCKS), dim3(256), 0, c.stream, (const float *)p,
+                           (const float *)q, c.R, part);
+    launch_reduce(c, part, DOT_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_TMP0);
+    int rc = launch_status("spm_inpfp");
+    if (rc) return rc;
+    return read_scalars(h, c.scal + SC_TMP0, 1, out);
+}
+
+int spm_nrmp(spm_ctx *h, const void *p, double *out)
+{
+    int rc = spm_inpfp(h, p, p, out);
+    if (rc == SPM_OK) *out = std::sqrt(*out);
+    return rc;
+}
+
+int spm_lin2(spm_ctx *h, void *r, double a1, const void *F1, double a2, const void *F2)
+{
+    CHECK_CTX(h);
+    if (!r || !F1 || !F2) return fail(SPM_EINVAL, "spm_lin2: null field");
+    const int nb = 2048;
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_lin2<double>, dim3(nb), dim3(256), 0, c.stream, (double *)r, a1, (const double *)F1,
+                           a2, (const double *)F2, c.R);
+    else
+        hipLaunchKernelGGL(k_lin2<float>, dim3(nb), dim3(256), 0, c.stream, (float *)r, a1, (const float *)F1, a2,
+                           (const float *)F2, c.R);
+    return launch_status("spm_lin2");
+}
+
+int spm_cplx_to_real(spm_ctx *h, const void *z, void *x)
+{
+    CHECK_CTX(h);
+    if (!z || !x) return fail(SPM_EINVAL, "null field");
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_cplx_to_real<double>, dim3(2048), dim3(256), 0, c.stream, (const C2<double> *)z,
+                           (double *)x, c.R);
+    else
+        hipLaunchKernelGGL(k_cplx_to_real<float>, dim3(2048), dim3(256), 0, c.stream, (const C2<float> *)z,
+                           (float *)x, c.R);
+    return launch_status("spm_cplx_to_real");
+}
+
+int spm_real_to_cplx(spm_ctx *h, const void *x, void *z)
+{
+    CHECK_CTX(h);
+    if (!z || !x) return fail(SPM_EINVAL, "null field");
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_real_to_cplx<double>, dim3(2048), dim3(256), 0, c.stream, (const double *)x,
+                           (C2<double> *)z, c.R);
+    else
+        hipLaunchKernelGGL(k_real_to_cplx<float>, dim3(2048), dim3(256), 0, c.stream, (const float *)x,
+                           (C2<float> *)z, c.R);
+    return launch_status("spm_real_to_cplx");
+}
+
+} // extern "C"

@@ -1,0 +1,77 @@
+// spm_ctx.h -- device-resident context of one N^3 grid (host-side C++).
+//
+// The reference keeps its state in process globals (Data.h:4-28, allocated by
+// memAllocation_local, Initialization.cpp:44-78).  Here one context owns the
+// device buffers for one grid size, model parameters and precision; every
+// operation runs on the context's HIP stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "grid_ops.h"
+
+namespace spm {
+
+struct GridIface;
+
+enum ScalarSlot { // device scalars (ctx.scal)
+    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_COUNT = 64
+};
+
+struct Ctx {
+    // grid and model (LPsystParameter, Initialization.cpp:7-42)
+    int N = 0, NH = 0, NHP = 0, prec = 0; // prec 0 = FP64, 1 = FP32
+    long R = 0, C = 0;                    // real points, half-spectrum entries (N*N*NHP)
+    size_t esz = 8;                       // sizeof(real)
+    double L = 26, kscale = 1, Tau = -0.0061, Gamma = 0.30, dt = 0.3;
+    hipStream_t stream = nullptr;
+    GridIface *g = nullptr;
+    // tables
+    void *tw = nullptr;      // exp(-2 pi i e / N), e < N, in the context precision
+    double *Gtab = nullptr;  // get_GK(kscale^2 n2 / L^2), n2 = |n|^2 = 0..3(N/2)^2
+    double *Itab = nullptr;  // 1 / (G + |Tau| + 0.02)  (get_ikt2_local)
+    int ntab = 0;
+    // work buffers
+    void *W = nullptr;       // spectrum-sized work array
+    void *U = nullptr;       // gradient-flow state spectrum
+    double *part = nullptr;  // block partial sums: nslots slots of part_slot doubles
+    long part_slot = 0;
+    int nslots = 0;
+    double *scal = nullptr;  // SC_COUNT device scalars
+    Ctl *ctl = nullptr;
+    double *hist = nullptr;  // 3 doubles per iteration: E, ||grad F||, res
+    long hist_cap = 0;
+    // block counts of the passes over one field
+    long nb_rows = 0, nb_j = 0, nb_i = 0;
+
+    double *slot(int s) { return part + (long)s * part_slot; }
+};
+
+// Transform and operator passes specialised for (precision, N).
+struct GridIface {
+    virtual ~GridIface() = default;
+    virtual void geometry(long &nb_rows, long &nb_j, long &nb_i, int NHP) = 0;
+    // S = FFT(x) / N^3 into the half spectrum S
+    virtual void r2c(Ctx &c, const void *x, void *S) = 0;
+    // x = IFFT(S) (unnormalised); S is preserved (W is used)
+    virtual void c2r(Ctx &c, const void *S, void *x) = 0;
+    // full complex N^3 arrays (the reference's fftw_complex layout):
+    // dir < 0: out = FFT(in) / N^3; dir > 0: out = IFFT(in); in may equal out
+    virtual void c2c(Ctx &c, const void *in, void *out, int dir) = 0;
+    // E1 (before the 1/2) and E2 sums of get_Ene into scal[SC_E1], scal[SC_E2]
+    virtual void energy(Ctx &c, const void *phi) = 0;
+    virtual void ngrad(Ctx &c, const void *phi, void *grad) = 0; // scal[SC_GN2] = nrmp^2
+    virtual void hv(Ctx &c, const void *phi, const void *v, void *hv) = 0;
+    virtual void precond(Ctx &c, void *w) = 0;
+    virtual void gf_init(Ctx &c, const void *phi) = 0;
+    virtual void gf_step(Ctx &c) = 0;
+};
+
+GridIface *make_grid(int N, int prec);
+
+// generic helpers (spm_capi.hip)
+void launch_reduce(Ctx &c, const double *part, long nblk, int nval, unsigned maxmask, double scale, double *out);
+void launch_gf_finalize(Ctx &c, const double *pA, long nA, const double *pC, long nC);
+
+} // namespace spm
