@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 512 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the workload is one 256^3 field); "
@@ -105,6 +107,12 @@ def main():
     N = args.N
     phi0 = seed_field(seeds, N)
 
+    if args.calib_copy:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     r64 = time_gpu(spm, torch, N, spm.FP64, args.steps, args.warmup, phi0)
     r32 = time_gpu(spm, torch, N, spm.FP32, args.steps, args.warmup, phi0)
 

@@ -3,7 +3,7 @@
  * pseudo-spectral Landau free-energy path of JKGroupCUDA/SPM-IQC.
  *
  * The reference has no FFI: its interface is the set of free C++ functions in
- * include/*.h operating on process globals (Data.h:4-28).  This header is the
+ * its include/ headers, operating on process globals (Data.h:4-28).  This header is the
  * plain-C boundary those functions map onto, one entry point per reference
  * function (cited beside each).  The C++ layer with the reference's own
  * signatures (spm_compat.h) is built on top of it.
@@ -117,6 +117,40 @@ int spm_gf_end(spm_ctx *c, void *phi, double *hist, int *iters);
 int spm_hv(spm_ctx *c, const void *phi, const void *v, void *hv);
 /* precond_camnew (LOBPCG.cpp:168-199), in place. */
 int spm_precond(spm_ctx *c, void *w);
+
+/* hiosd_lobpcg_initialization (LOBPCG.cpp:10-109): V (k fields, contiguous)
+ * <- the random start block (k fields; the reference draws it from Eigen's
+ * MatrixXd::Random) with each field's mean removed, orthonormalised in order
+ * under inpfp.  start may equal V.  1 <= k <= 8. */
+int spm_lobpcg_init(spm_ctx *c, const void *start, void *V, int k);
+/* hiosd_lobpcg_inip (LOBPCG.cpp:207-487): maxiter iterations on V (k fields,
+ * in place) for the Hessian at phi.  ritz (host, k doubles, may be NULL)
+ * receives the lowest k Ritz values of the last Rayleigh-Ritz step (the
+ * reference only prints them, LOBPCG.cpp:406-410).  1 <= k <= 8. */
+int spm_lobpcg(spm_ctx *c, const void *phi, void *V, int k, int maxiter, double *ritz);
+/* The block Gram of the Rayleigh-Ritz step (Pn = UUm * YYm^T / N^3,
+ * LOBPCG.cpp:336-358): out[a*mb + b] = inpfp(X_a, Y_b) for the contiguous
+ * blocks X (ma fields) and Y (mb fields), 1 <= ma, mb <= 32; FP64
+ * accumulation on the matrix cores; out is a host array. */
+int spm_gram(spm_ctx *c, const void *X, int ma, const void *Y, int mb, double *out);
+/* rayleighq (BasicOperators.cpp:174-185): <v, H v> / <v, v>. */
+int spm_rayleighq(spm_ctx *c, const void *phi, const void *v, double *out);
+
+/* --- SPM */
+/* get_Tv (SPM.cpp:15-32): v = (phi1 - phi0) / nrmp(phi1 - phi0). */
+int spm_get_tv(spm_ctx *c, const void *phi0, const void *phi1, void *v);
+/* Natural_Forces (SPM.cpp:34-99) and MinMax_Forces (SPM.cpp:103-168; not
+ * declared in the reference's SPM.h): one semi-implicit force step on x in
+ * place along the unit tangent v. */
+int spm_natural_force(spm_ctx *c, void *x, const void *v);
+int spm_minmax_force(spm_ctx *c, void *x, const void *v);
+/* Spring_Pairing (SPM.cpp:170-263) on the replicas phi0, phi1 in place:
+ * iterates while res > tol and fewer than maxiter iterations (the reference
+ * driver: 30000, 1e-9, main_SPM.cpp:122).  *iters, *res receive the count
+ * and the last residual max_i nrmp(ngrad(phi_i)); res_hist (host, may be
+ * NULL, maxiter doubles) the residual of every iteration. */
+int spm_spring_pairing(spm_ctx *c, void *phi0, void *phi1, int maxiter, double tol, int *iters, double *res,
+                       double *res_hist);
 
 /* --- BasicOperators: inpfp / nrmp (BasicOperators.cpp:132-172), FP64 sums. */
 int spm_inpfp(spm_ctx *c, const void *p, const void *q, double *out);

@@ -699,15 +699,10 @@ int lpo_lobpcg_inip(lpo_ctx *c, const lpo_cplx *x, lpo_cplx *Vio, int k, int max
         for (int a = 0; a < m; ++a)
             for (int b = 0; b < m; ++b) Pw[a * m + b] = (Pn[b * m + a] + Pn[a * m + b]) * 0.5 / (int)R;
         lpo_sym_eig(m, Pw, val, Pz);
-        for (int i = 0; i < k; ++i) { /* Vp <- V, Up <- U (:391-400) */
-            for (long j = 0; j < R; ++j) {
-                Vp[i][j][0] = V[i][j][0]; Vp[i][j][1] = 0.0;
-                Up[i][j][0] = U[i][j][0]; Up[i][j][1] = 0.0;
-            }
-        }
         if (ritz)
             for (int i = 0; i < k; ++i) ritz[i] = val[i];
-        /* Vm = eta1^T * UUm (:402-403) */
+        /* Vm = eta1^T * UUm (:402-403).  UUm is a copy taken before the
+         * Vp <- V copy below (get_mat, :341), so it is formed first here. */
 #pragma omp parallel for num_threads(c->nthreads) schedule(static)
         for (long j = 0; j < R; ++j)
             for (int i = 0; i < k; ++i) {
@@ -715,6 +710,12 @@ int lpo_lobpcg_inip(lpo_ctx *c, const lpo_cplx *x, lpo_cplx *Vio, int k, int max
                 for (int a = 0; a < m; ++a) s += Pz[a * m + i] * UU[a][j][0];
                 Vm[(long)i * R + j] = s;
             }
+        for (int i = 0; i < k; ++i) { /* Vp <- V, Up <- U (:391-400) */
+            for (long j = 0; j < R; ++j) {
+                Vp[i][j][0] = V[i][j][0]; Vp[i][j][1] = 0.0;
+                Up[i][j][0] = U[i][j][0]; Up[i][j][1] = 0.0;
+            }
+        }
         for (int i = 0; i < k; ++i)
             for (long j = 0; j < R; ++j) {
                 V[i][j][0] = Vm[(long)i * R + j];

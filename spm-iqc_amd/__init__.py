@@ -71,6 +71,14 @@ SIGNATURES = {
     "spm_gf_steps": (_i, [_c, _i]),
     "spm_gf_end": (_i, [_c, _c, _c, _pi]),
     "spm_hv": (_i, [_c, _c, _c, _c]),
+    "spm_lobpcg_init": (_i, [_c, _c, _c, _i]),
+    "spm_lobpcg": (_i, [_c, _c, _c, _i, _i, _c]),
+    "spm_rayleighq": (_i, [_c, _c, _c, _pd]),
+    "spm_gram": (_i, [_c, _c, _i, _c, _i, _c]),
+    "spm_get_tv": (_i, [_c, _c, _c, _c]),
+    "spm_natural_force": (_i, [_c, _c, _c]),
+    "spm_minmax_force": (_i, [_c, _c, _c]),
+    "spm_spring_pairing": (_i, [_c, _c, _c, _i, _d, _pi, _pd, _c]),
     "spm_precond": (_i, [_c, _c]),
     "spm_inpfp": (_i, [_c, _c, _c, _pd]),
     "spm_nrmp": (_i, [_c, _c, _pd]),
@@ -275,6 +283,69 @@ class Grid:
         self._stream()
         _check(lib().spm_precond(self.h, self._real(w)), "spm_precond")
         return w
+
+    def _block(self, t, k=None):
+        k = t.shape[0] if k is None else k
+        return self._ptr(t, self.dtype, (k, self.N, self.N, self.N))
+
+    def lobpcg_init(self, start, V=None):
+        """hiosd_lobpcg_initialization (LOBPCG.cpp:10-109) from a (k, N, N, N) start block."""
+        self._stream()
+        V = self.torch.empty_like(start) if V is None else V
+        _check(lib().spm_lobpcg_init(self.h, self._block(start), self._block(V), int(start.shape[0])),
+               "spm_lobpcg_init")
+        return V
+
+    def lobpcg(self, phi, V, maxiter=21):
+        """hiosd_lobpcg_inip (LOBPCG.cpp:207-487) on V in place; returns the last lowest-k Ritz values."""
+        self._stream()
+        k = int(V.shape[0])
+        ritz = np.zeros(k)
+        _check(lib().spm_lobpcg(self.h, self._real(phi), self._block(V), k, int(maxiter), ritz.ctypes.data_as(_c)),
+               "spm_lobpcg")
+        return ritz
+
+    def gram(self, X, Y):
+        """inpfp of every pair of fields of the blocks X (ma, N, N, N), Y (mb, N, N, N)."""
+        self._stream()
+        ma, mb = int(X.shape[0]), int(Y.shape[0])
+        out = np.zeros((ma, mb))
+        _check(lib().spm_gram(self.h, self._block(X), ma, self._block(Y), mb, out.ctypes.data_as(_c)), "spm_gram")
+        return out
+
+    def rayleighq(self, phi, v) -> float:
+        self._stream()
+        out = ctypes.c_double()
+        _check(lib().spm_rayleighq(self.h, self._real(phi), self._real(v), ctypes.byref(out)), "spm_rayleighq")
+        return out.value
+
+    # -- SPM ------------------------------------------------------------------
+    def get_tv(self, phi0, phi1, v=None):
+        self._stream()
+        v = self.real_field() if v is None else v
+        _check(lib().spm_get_tv(self.h, self._real(phi0), self._real(phi1), self._real(v)), "spm_get_tv")
+        return v
+
+    def natural_force(self, x, v):
+        self._stream()
+        _check(lib().spm_natural_force(self.h, self._real(x), self._real(v)), "spm_natural_force")
+        return x
+
+    def minmax_force(self, x, v):
+        self._stream()
+        _check(lib().spm_minmax_force(self.h, self._real(x), self._real(v)), "spm_minmax_force")
+        return x
+
+    def spring_pairing(self, phi0, phi1, maxiter, tol):
+        """Spring_Pairing (SPM.cpp:170-263) in place; returns (iters, res, res_hist)."""
+        self._stream()
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        hist = np.zeros(max(int(maxiter), 1))
+        _check(lib().spm_spring_pairing(self.h, self._real(phi0), self._real(phi1), int(maxiter), float(tol),
+                                        ctypes.byref(it), ctypes.byref(res), hist.ctypes.data_as(_c)),
+               "spm_spring_pairing")
+        return it.value, res.value, hist[: it.value]
 
     # -- BasicOperators -------------------------------------------------------
     def inpfp(self, p, q) -> float:

@@ -1,0 +1,39 @@
+// block_ops.h -- operations on small blocks of fields (host side): the Gram
+// matrix X^T Y on the matrix cores, linear combinations, means.  Used by the
+// LOBPCG and SPM drivers.
+#pragma once
+#include "capi_internal.h"
+
+namespace spm {
+
+constexpr int BLK_MAX = 32; // fields per block operand
+
+struct FieldPtrs {
+    const void *p[BLK_MAX];
+};
+struct FieldPtrsW {
+    void *p[BLK_MAX];
+};
+
+// out[a*mb + b] = inpfp(X_a, Y_b) = (1/R) sum_r X_a[r] Y_b[r]  (FP64 accumulation;
+// BasicOperators.cpp:132-143 for every pair), host array, synchronises.
+int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, int mb, double *out);
+// out_i = sum_a coef[a*nout + i] in_a + shift[i] (shift may be NULL); in-place allowed
+int block_combine(spm_ctx *h, const void *const *in, int nin, void *const *out, int nout, const double *coef,
+                  const double *shift);
+// out[a] = mean of X_a (host), synchronises
+int block_means(spm_ctx *h, const void *const *X, int m, double *out);
+// copy field src -> dst
+int field_copy(spm_ctx *h, void *dst, const void *src);
+int field_zero(spm_ctx *h, void *dst);
+
+// Orthonormalise the fields Q[0..nq) in place, in order, as the reference's
+// modified Gram-Schmidt does (Orthoi_v2V / Orthoi_Vi + nrmp + scaling,
+// BasicOperators.cpp:229-255): first against the orthonormal fields
+// B[0..nb) (which may include zero fields), then each against the earlier
+// ones.  Field i is deflated (set to zero, keep[i] = false) when its norm
+// after projection is <= tol (tol < 0: never).  Carried out as two rounds of
+// classical projection and Cholesky QR on FP64 Gram matrices (CholQR2).
+int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb, double tol, bool *keep);
+
+} // namespace spm

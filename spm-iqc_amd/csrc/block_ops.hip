@@ -1,0 +1,364 @@
+// block_ops.hip -- Gram matrices on the matrix cores, linear combinations and
+// means of small blocks of fields, and the CholQR2 orthonormalisation built
+// on them.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "block_ops.h"
+
+namespace spm {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int GRAM_BLOCKS = 512;  // blocks of 4 waves; fixed, so the summation order is fixed
+constexpr int GRAM_TILE = 32;     // the Gram is computed as a padded 32 x 32 block
+
+// Each wave accumulates C = X^T Y over its 16-point chunks with
+// v_mfma_f64_16x16x4f64 (A: lane l holds A[l%16][l/16]; B: B[l/16][l%16];
+// D: lane l holds D[l/16 + 4v][l%16], checked on gfx950).  The K index of a chunk is permuted so
+// that lane l reads 4 consecutive points r0 + 4(l/16) .. +3 of its field in
+// one 32-byte (FP64) or 16-byte (FP32) load; the sum over K is unchanged.
+template <typename T, int TA, int TB>
+__global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, int mb, long R,
+                                             double *__restrict__ part)
+{
+    __shared__ double sh[4][TA * TB * 4][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    d4 acc[TA][TB];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) acc[ta][tb] = d4{0.0, 0.0, 0.0, 0.0};
+    const T *xa[TA], *yb[TB];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) xa[ta] = (ta * 16 + row < ma) ? (const T *)X.p[ta * 16 + row] : nullptr;
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) yb[tb] = (tb * 16 + row < mb) ? (const T *)Y.p[tb * 16 + row] : nullptr;
+    const long nchunk = R / 16;
+    for (long q = (long)blockIdx.x * 4 + wid; q < nchunk; q += (long)gridDim.x * 4) {
+        const long r0 = q * 16 + 4 * kq;
+        double a[TA][4], b[TB][4];
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) {
+            if (xa[ta]) {
+                if constexpr (sizeof(T) == 8) {
+                    const double2 u = *(const double2 *)(xa[ta] + r0), w = *(const double2 *)(xa[ta] + r0 + 2);
+                    a[ta][0] = u.x; a[ta][1] = u.y; a[ta][2] = w.x; a[ta][3] = w.y;
+                } else {
+                    const float4 u = *(const float4 *)(xa[ta] + r0);
+                    a[ta][0] = u.x; a[ta][1] = u.y; a[ta][2] = u.z; a[ta][3] = u.w;
+                }
+            } else {
+                a[ta][0] = a[ta][1] = a[ta][2] = a[ta][3] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+            if (yb[tb]) {
+                if constexpr (sizeof(T) == 8) {
+                    const double2 u = *(const double2 *)(yb[tb] + r0), w = *(const double2 *)(yb[tb] + r0 + 2);
+                    b[tb][0] = u.x; b[tb][1] = u.y; b[tb][2] = w.x; b[tb][3] = w.y;
+                } else {
+                    const float4 u = *(const float4 *)(yb[tb] + r0);
+                    b[tb][0] = u.x; b[tb][1] = u.y; b[tb][2] = u.z; b[tb][3] = u.w;
+                }
+            } else {
+                b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+                for (int tb = 0; tb < TB; ++tb)
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], b[tb][s], acc[ta][tb], 0, 0, 0);
+    }
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sh[wid][(ta * TB + tb) * 4 + v][lane] = acc[ta][tb][v];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int e = (ta * TB + tb) * 4 + v;
+                    const double s = ((sh[0][e][lane] + sh[1][e][lane]) + sh[2][e][lane]) + sh[3][e][lane];
+                    const int ai = ta * 16 + kq + 4 * v, bj = tb * 16 + row;
+                    part[(long)blockIdx.x * GRAM_TILE * GRAM_TILE + ai * GRAM_TILE + bj] = s;
+                }
+    }
+}
+
+// out[e] = scale * sum over blocks of part[b][e], in block order
+__global__ void __launch_bounds__(256) k_gram_reduce(const double *__restrict__ part, int nblk, double scale,
+                                                    double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= GRAM_TILE * GRAM_TILE) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(long)b * GRAM_TILE * GRAM_TILE + e];
+    out[e] = s * scale;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_combine(FieldPtrs in, int nin, FieldPtrsW out, int nout,
+                                                const double *__restrict__ coef, const double *__restrict__ shift,
+                                                long R)
+{
+    __shared__ double cs[BLK_MAX * BLK_MAX];
+    __shared__ double ss[BLK_MAX];
+    for (int e = threadIdx.x; e < nin * nout; e += blockDim.x) cs[e] = coef[e];
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) ss[e] = shift ? shift[e] : 0.0;
+    __syncthreads();
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (long)gridDim.x * blockDim.x) {
+        double x[BLK_MAX];
+#pragma unroll
+        for (int a = 0; a < BLK_MAX; ++a) x[a] = a < nin ? (double)((const T *)in.p[a])[r] : 0.0;
+#pragma unroll 4
+        for (int i = 0; i < nout; ++i) {
+            double s = ss[i];
+#pragma unroll
+            for (int a = 0; a < BLK_MAX; ++a)
+                if (a < nin) s += cs[a * nout + i] * x[a];
+            ((T *)out.p[i])[r] = (T)s;
+        }
+    }
+}
+
+constexpr int SUM_BLOCKS = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sums(FieldPtrs X, long R, double *__restrict__ part)
+{
+    __shared__ double red[16];
+    const T *x = (const T *)X.p[blockIdx.y];
+    double s = 0.0;
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (long)gridDim.x * blockDim.x)
+        s += (double)x[r];
+    double v[1] = {s};
+    block_reduce<1, 0u>(v, red);
+    if (threadIdx.x == 0) part[(long)blockIdx.y * gridDim.x + blockIdx.x] = v[0];
+}
+
+__global__ void k_sums_reduce(const double *__restrict__ part, int nblk, int m, double scale,
+                              double *__restrict__ out)
+{
+    const int a = threadIdx.x;
+    if (a >= m) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(long)a * nblk + b];
+    out[a] = s * scale;
+}
+
+// ---------------------------------------------------------------- host side
+
+static int ensure_bscr(spm_ctx *h)
+{
+    const size_t need = sizeof(double) * ((size_t)GRAM_BLOCKS * GRAM_TILE * GRAM_TILE + 4 * 4096);
+    if (h->bscr_bytes >= need) return SPM_OK;
+    if (h->bscr) {
+        HIPCHK(hipStreamSynchronize(h->c.stream));
+        HIPCHK(hipFree(h->bscr));
+        h->bscr = nullptr;
+    }
+    RCHK(spm_dev_alloc(h, (void **)&h->bscr, need));
+    h->bscr_bytes = need;
+    return SPM_OK;
+}
+// scratch regions inside bscr
+static double *gram_part(spm_ctx *h) { return h->bscr; }
+static double *small_dev(spm_ctx *h) { return h->bscr + (size_t)GRAM_BLOCKS * GRAM_TILE * GRAM_TILE; }
+
+int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, int mb, double *out)
+{
+    Ctx &c = h->c;
+    if (ma < 1 || mb < 1 || ma > GRAM_TILE || mb > GRAM_TILE) return spm_fail(SPM_EINVAL, "block_gram: 1..32 fields");
+    RCHK(ensure_bscr(h));
+    FieldPtrs xp{}, yp{};
+    for (int a = 0; a < ma; ++a) xp.p[a] = X[a];
+    for (int b = 0; b < mb; ++b) yp.p[b] = Y[b];
+    const int ta = ma > 16 ? 2 : 1, tb = mb > 16 ? 2 : 1;
+    const int nblk = (int)std::min<long>(GRAM_BLOCKS, std::max<long>(1, c.R / 64));
+    double *part = gram_part(h);
+#define GRAM_LAUNCH(T, A, B)                                                                                   \
+    hipLaunchKernelGGL((k_gram<T, A, B>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
+    if (c.prec == SPM_FP64) {
+        if (ta == 1 && tb == 1) GRAM_LAUNCH(double, 1, 1);
+        else if (ta == 1) GRAM_LAUNCH(double, 1, 2);
+        else if (tb == 1) GRAM_LAUNCH(double, 2, 1);
+        else GRAM_LAUNCH(double, 2, 2);
+    } else {
+        if (ta == 1 && tb == 1) GRAM_LAUNCH(float, 1, 1);
+        else if (ta == 1) GRAM_LAUNCH(float, 1, 2);
+        else if (tb == 1) GRAM_LAUNCH(float, 2, 1);
+        else GRAM_LAUNCH(float, 2, 2);
+    }
+#undef GRAM_LAUNCH
+    double *dout = small_dev(h);
+    hipLaunchKernelGGL(k_gram_reduce, dim3(4), dim3(256), 0, c.stream, part, nblk, 1.0 / (double)c.R, dout);
+    RCHK(spm_launch_status("block_gram"));
+    std::vector<double> full(GRAM_TILE * GRAM_TILE);
+    HIPCHK(hipMemcpyAsync(full.data(), dout, sizeof(double) * full.size(), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    for (int a = 0; a < ma; ++a)
+        for (int b = 0; b < mb; ++b) out[a * mb + b] = full[a * GRAM_TILE + b];
+    return SPM_OK;
+}
+
+int block_combine(spm_ctx *h, const void *const *in, int nin, void *const *out, int nout, const double *coef,
+                  const double *shift)
+{
+    Ctx &c = h->c;
+    if (nin < 1 || nout < 1 || nin > BLK_MAX || nout > BLK_MAX) return spm_fail(SPM_EINVAL, "block_combine");
+    RCHK(ensure_bscr(h));
+    // coefficients: uploaded after the stream has drained (the scratch may be in use by queued work)
+    double *dc = small_dev(h) + GRAM_TILE * GRAM_TILE;
+    std::vector<double> buf(nin * nout + nout, 0.0);
+    std::memcpy(buf.data(), coef, sizeof(double) * nin * nout);
+    if (shift) std::memcpy(buf.data() + nin * nout, shift, sizeof(double) * nout);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipMemcpy(dc, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+    FieldPtrs ip{};
+    FieldPtrsW op{};
+    for (int a = 0; a < nin; ++a) ip.p[a] = in[a];
+    for (int i = 0; i < nout; ++i) op.p[i] = out[i];
+    const int nb = (int)std::min<long>(2048, (c.R + 255) / 256);
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_combine<double>, dim3(nb), dim3(256), 0, c.stream, ip, nin, op, nout, dc,
+                           shift ? dc + nin * nout : nullptr, c.R);
+    else
+        hipLaunchKernelGGL(k_combine<float>, dim3(nb), dim3(256), 0, c.stream, ip, nin, op, nout, dc,
+                           shift ? dc + nin * nout : nullptr, c.R);
+    return spm_launch_status("block_combine");
+}
+
+int block_means(spm_ctx *h, const void *const *X, int m, double *out)
+{
+    Ctx &c = h->c;
+    if (m < 1 || m > BLK_MAX) return spm_fail(SPM_EINVAL, "block_means");
+    RCHK(ensure_bscr(h));
+    FieldPtrs xp{};
+    for (int a = 0; a < m; ++a) xp.p[a] = X[a];
+    double *part = gram_part(h);
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_sums<double>, dim3(SUM_BLOCKS, m), dim3(256), 0, c.stream, xp, c.R, part);
+    else
+        hipLaunchKernelGGL(k_sums<float>, dim3(SUM_BLOCKS, m), dim3(256), 0, c.stream, xp, c.R, part);
+    double *dout = small_dev(h);
+    hipLaunchKernelGGL(k_sums_reduce, dim3(1), dim3(64), 0, c.stream, part, SUM_BLOCKS, m, 1.0 / (double)c.R, dout);
+    RCHK(spm_launch_status("block_means"));
+    HIPCHK(hipMemcpyAsync(h->h_pin, dout, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    std::memcpy(out, h->h_pin, sizeof(double) * m);
+    return SPM_OK;
+}
+
+int field_copy(spm_ctx *h, void *dst, const void *src)
+{
+    if (dst == src) return SPM_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, (size_t)h->c.R * h->c.esz, hipMemcpyDeviceToDevice, h->c.stream));
+    return SPM_OK;
+}
+
+int field_zero(spm_ctx *h, void *dst)
+{
+    HIPCHK(hipMemsetAsync(dst, 0, (size_t)h->c.R * h->c.esz, h->c.stream));
+    return SPM_OK;
+}
+
+// Cholesky factor of the Gram S (n x n, row-major) with the reference's
+// sequential deflation: column i is dropped when its remaining norm is <= tol.
+// Returns Rinv (upper triangular, dropped columns zero) so that Q R^{-1} has
+// orthonormal kept columns.
+static void chol_inv(int n, const double *S, double tol, bool deflate, bool *keep, double *Rinv)
+{
+    std::vector<double> Rm(n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        keep[i] = true;
+        double d = S[i * n + i];
+        for (int l = 0; l < i; ++l) d -= Rm[l * n + i] * Rm[l * n + i];
+        if (deflate ? !(d > tol * tol) : !(d > 0.0)) {
+            keep[i] = false;
+            continue;
+        }
+        const double rii = std::sqrt(d);
+        Rm[i * n + i] = rii;
+        for (int j = i + 1; j < n; ++j) {
+            double s = S[i * n + j];
+            for (int l = 0; l < i; ++l) s -= Rm[l * n + i] * Rm[l * n + j];
+            Rm[i * n + j] = s / rii;
+        }
+    }
+    // Rinv: solve R X = I on the kept index set (back substitution per column)
+    std::fill(Rinv, Rinv + n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        if (!keep[j]) continue;
+        for (int i = j; i >= 0; --i) {
+            if (!keep[i]) continue;
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int l = i + 1; l <= j; ++l)
+                if (keep[l]) s -= Rm[i * n + l] * Rinv[l * n + j];
+            Rinv[i * n + j] = s / Rm[i * n + i];
+        }
+    }
+}
+
+int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb, double tol, bool *keep)
+{
+    if (nq < 1) return SPM_OK;
+    if (nq > 16 || nb > 16) return spm_fail(SPM_EINVAL, "block_ortho: at most 16 + 16 fields");
+    std::vector<const void *> qin(Q, Q + nq);
+    // two rounds of Q <- Q - B (B^T Q)
+    if (nb > 0) {
+        for (int round = 0; round < 2; ++round) {
+            std::vector<double> P(nb * nq);
+            RCHK(block_gram(h, B, nb, qin.data(), nq, P.data()));
+            // out_i = Q_i - sum_j P[j][i] B_j : inputs [Q_0..Q_{nq-1}, B_0..B_{nb-1}]
+            std::vector<const void *> in(nq + nb);
+            std::vector<double> coef((nq + nb) * nq, 0.0);
+            for (int i = 0; i < nq; ++i) {
+                in[i] = Q[i];
+                coef[i * nq + i] = 1.0;
+            }
+            for (int j = 0; j < nb; ++j) {
+                in[nq + j] = B[j];
+                for (int i = 0; i < nq; ++i) coef[(nq + j) * nq + i] = -P[j * nq + i];
+            }
+            RCHK(block_combine(h, in.data(), nq + nb, Q, nq, coef.data(), nullptr));
+        }
+    }
+    // CholQR with the reference's deflation, then a second CholQR on the kept fields
+    std::vector<double> S(nq * nq), Rinv(nq * nq);
+    RCHK(block_gram(h, qin.data(), nq, qin.data(), nq, S.data()));
+    chol_inv(nq, S.data(), tol, tol >= 0.0, keep, Rinv.data());
+    RCHK(block_combine(h, qin.data(), nq, Q, nq, Rinv.data(), nullptr));
+    std::vector<int> kept;
+    for (int i = 0; i < nq; ++i)
+        if (keep[i]) kept.push_back(i);
+    const int nk = (int)kept.size();
+    if (nk > 0) {
+        std::vector<const void *> kin(nk);
+        std::vector<void *> kout(nk);
+        for (int a = 0; a < nk; ++a) {
+            kin[a] = Q[kept[a]];
+            kout[a] = Q[kept[a]];
+        }
+        std::vector<double> S2(nk * nk), R2(nk * nk);
+        bool k2[BLK_MAX];
+        RCHK(block_gram(h, kin.data(), nk, kin.data(), nk, S2.data()));
+        chol_inv(nk, S2.data(), -1.0, false, k2, R2.data());
+        RCHK(block_combine(h, kin.data(), nk, kout.data(), nk, R2.data(), nullptr));
+    }
+    return SPM_OK;
+}
+
+} // namespace spm

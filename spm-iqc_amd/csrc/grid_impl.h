@@ -3,6 +3,8 @@
 #pragma once
 #include "spm_ctx.h"
 
+#include <algorithm>
+
 namespace spm {
 
 template <typename T, int N> struct GridImpl final : GridIface {
@@ -18,23 +20,24 @@ template <typename T, int N> struct GridImpl final : GridIface {
     }
 
     // ---------------------------------------------------------- launchers
-    template <bool IN_SPEC, bool OUT_SPEC, class Op>
-    static void rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl = nullptr)
+    template <bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
+    static void rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl = nullptr,
+                     Pro pro = Pro{})
     {
         const long nb = (NPAIRS + G::PP - 1) / G::PP;
-        hipLaunchKernelGGL((k_rows<T, N, IN_SPEC, OUT_SPEC, Op>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0,
-                           c.stream, in, (CT *)out, c.NHP, NPAIRS, (const CT *)c.tw, op, part, ctl);
+        hipLaunchKernelGGL((k_rows<T, N, IN_SPEC, OUT_SPEC, Op, Pro>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0,
+                           c.stream, in, (CT *)out, c.NHP, NPAIRS, (const CT *)c.tw, op, part, ctl, pro);
     }
     // pencils along j inside each i plane; row length nhp, valid nh
-    template <bool FWD, bool INV, bool STORE, class SOp>
+    template <bool FWD, bool INV, bool STORE, class SOp, class Pro = ProNone>
     static void cols_j(Ctx &c, const void *src, void *dst, SOp op, double *part, const Ctl *ctl = nullptr,
-                       int nh = -1, int nhp = -1)
+                       int nh = -1, int nhp = -1, Pro pro = Pro{})
     {
         if (nh < 0) { nh = c.NH; nhp = c.NHP; }
         const long nb = (long)N * ((nhp + G::B - 1) / G::B);
-        hipLaunchKernelGGL((k_cols<T, N, FWD, INV, STORE, SOp>), dim3((unsigned)nb), dim3(G::COL_THREADS), 0,
+        hipLaunchKernelGGL((k_cols<T, N, FWD, INV, STORE, SOp, Pro>), dim3((unsigned)nb), dim3(G::COL_THREADS), 0,
                            c.stream, (const CT *)src, (CT *)dst, (long)nhp, (long)nhp, (long)N * nhp, nh, nhp, 0,
-                           (const CT *)c.tw, op, part, ctl);
+                           (const CT *)c.tw, op, part, ctl, pro);
     }
     // pencils along i over the whole array
     template <bool FWD, bool INV, bool STORE, class SOp>
@@ -131,6 +134,15 @@ template <typename T, int N> struct GridImpl final : GridIface {
         rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)w}, nullptr);
     }
 
+    void force(Ctx &c, void *x, const void *v, const double *s, double coef) override
+    {
+        rows<false, true>(c, x, c.W, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+    }
+
     // ---------------------------------------------------------- gradient flow
     // State: U = u_F (normalised spectrum of phi).  W carries the spectrum of
     // f(phi) between passes.  Slot 2 holds the E2 sums of the current phi
@@ -147,12 +159,14 @@ template <typename T, int N> struct GridImpl final : GridIface {
     // replayed from a graph) past convergence.
     void gf_step(Ctx &c) override
     {
+        const int kf = (int)std::min<long>(64, c.nb_j);
         cols_i<true, true, true>(c, c.W, c.W,
-                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, c.Gtab, (CT *)c.U}, c.slot(3),
-                                 c.ctl);
-        launch_gf_finalize(c, c.slot(3), c.nb_i, c.slot(2), c.nb_rows);
-        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
-        rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr}, c.slot(2), c.ctl);
+                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U},
+                                 c.slot(3), c.ctl);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl, -1, -1,
+                                  ProGradFlowReduce{c.slot(3), c.nb_i, c.slot(2), c.nb_rows, c.slot(4), kf});
+        rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr}, c.slot(2), c.ctl,
+                         ProGradFlowFinalize{c.slot(4), kf, (double)c.R, c.ctl, c.hist});
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
     }
 };

@@ -83,7 +83,7 @@ template <typename T> struct OpGradFlowRow {
     __device__ __forceinline__ T operator()(T x, long g, double *acc) const
     {
         const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
-        acc[0] += -0.5 * Tau * u2 - Gamma * u3 / 3 + 0.25 * u4;
+        acc[0] += -0.5 * Tau * u2 - Gamma * u3 * (1.0 / 3.0) + 0.25 * u4;
         if (phi_out) phi_out[g] = x;
         return (T)(Tau * u + Gamma * u2 - u3);
     }
@@ -97,7 +97,7 @@ template <typename T> struct OpDensity {
     __device__ __forceinline__ T operator()(T x, long, double *acc) const
     {
         const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
-        acc[0] += -0.5 * Tau * u2 - Gamma * u3 / 3 + 0.25 * u4;
+        acc[0] += -0.5 * Tau * u2 - Gamma * u3 * (1.0 / 3.0) + 0.25 * u4;
         return x;
     }
 };
@@ -168,6 +168,23 @@ template <typename T> struct OpHvOut {
     }
 };
 
+// Natural_Forces / MinMax_Forces input (SPM.cpp:53-86 with beta = 1): the
+// three transforms FFT(x) + FFT(nln - mean) - FFT(coef <f, v> v) of the
+// reference are one transform of y = x + nln(x) - coef s v, s = <f, v> (a
+// device scalar); the mean of nln only moves the DC, which is set to zero.
+template <typename T> struct OpForceIn {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma, coef;
+    const T *v;
+    const double *s;
+    __device__ __forceinline__ T operator()(T x, long g, double *) const
+    {
+        const double u = (double)x, u2 = u * u, u3 = u2 * u;
+        return (T)(u + (Tau * u + Gamma * u2 - u3) - coef * (*s) * (double)v[g]);
+    }
+};
+
 // ======================================================================
 // Spectral operators applied inside the i-axis column pass.
 // op(v, s, i, j, kz, acc): v = (unnormalised forward) value at spectrum index
@@ -176,14 +193,19 @@ template <typename T> struct OpHvOut {
 struct SOpNone {
     static constexpr int NRED = 0;
     static constexpr unsigned MAXM = 0;
-    template <typename T> __device__ __forceinline__ void operator()(C2<T> &, long, int, int, int, double *) const {}
+    static constexpr bool PRELOAD = false, FINALIZE = false;
+    template <typename T>
+    __device__ __forceinline__ void operator()(C2<T> &, C2<T>, long, int, int, int, double *) const
+    {
+    }
 };
 
 template <typename T> struct SOpScale { // r2c completion: v *= sc
     static constexpr int NRED = 0;
     static constexpr unsigned MAXM = 0;
+    static constexpr bool PRELOAD = false, FINALIZE = false;
     double sc;
-    __device__ __forceinline__ void operator()(C2<T> &v, long, int, int, int, double *) const
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T>, long, int, int, int, double *) const
     {
         v.x = (T)((double)v.x * sc);
         v.y = (T)((double)v.y * sc);
@@ -195,9 +217,10 @@ template <typename T> struct SOpScale { // r2c completion: v *= sc
 template <typename T, int N> struct SOpMulTab {
     static constexpr int NRED = 0;
     static constexpr unsigned MAXM = 0;
+    static constexpr bool PRELOAD = false, FINALIZE = false;
     double sc;
     const double *tab;
-    __device__ __forceinline__ void operator()(C2<T> &v, long, int i, int j, int kz, double *) const
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T>, long, int i, int j, int kz, double *) const
     {
         const int a = kfold(i, N), b = kfold(j, N);
         const int n2 = a * a + b * b + kz * kz;
@@ -214,9 +237,10 @@ template <int N> __device__ __forceinline__ double hweight(int kz) { return (kz 
 template <typename T, int N> struct SOpEnergy {
     static constexpr int NRED = 1;
     static constexpr unsigned MAXM = 0;
+    static constexpr bool PRELOAD = false, FINALIZE = false;
     double sc;
     const double *G;
-    __device__ __forceinline__ void operator()(C2<T> &v, long, int i, int j, int kz, double *acc) const
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T>, long, int i, int j, int kz, double *acc) const
     {
         const int a = kfold(i, N), b = kfold(j, N);
         const double re = (double)v.x * sc, im = (double)v.y * sc;
@@ -229,21 +253,26 @@ template <typename T, int N> struct SOpEnergy {
 // Sums, for the pre-step state: w G |U|^2 (2 E1), w |G U - f_hat|^2 over
 // k != 0 (||grad F||^2 by Parseval), and max |Re(U' - U)| (the reference's
 // residual).  v becomes U' (then inverse-transformed by the same pass).
+// U is loaded at the start of the pass (PRELOAD) so its latency overlaps the
+// forward transform.  G and D = 1 / (1 + dt G) come from one table entry
+// (one gather, and a multiply instead of two FP64 divides per entry).
 template <typename T, int N> struct SOpGradFlowStep {
     static constexpr int NRED = 3;
     static constexpr unsigned MAXM = 4u; // acc[2] is a max
+    static constexpr bool PRELOAD = true, FINALIZE = false;
     double sc, dt;
-    const double *G;
+    const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
-    __device__ __forceinline__ void operator()(C2<T> &v, long s, int i, int j, int kz, double *acc) const
+    __device__ __forceinline__ const C2<T> *pre_src() const { return U; }
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T> u, long s, int i, int j, int kz, double *acc) const
     {
         const int a = kfold(i, N), b = kfold(j, N);
         const int n2 = a * a + b * b + kz * kz;
-        const double g = G[n2], w = hweight<N>(kz);
-        const C2<T> u = U[s];
+        const C2<T> gd = GD[n2];
+        const double g = (double)gd.x, d = (double)gd.y, w = hweight<N>(kz);
         const double fr = (double)v.x * sc, fi = (double)v.y * sc;
         const double ur = (double)u.x, ui = (double)u.y;
-        double nr = (dt * fr + ur) / (1 + dt * g), ni = (dt * fi + ui) / (1 + dt * g);
+        double nr = (dt * fr + ur) * d, ni = (dt * fi + ui) * d;
         if (n2 == 0) {
             nr = 0.0;
             ni = 0.0;
@@ -256,6 +285,83 @@ template <typename T, int N> struct SOpGradFlowStep {
         acc[2] = fmax(acc[2], fabs((double)un.x - ur));
         U[s] = un;
         v = un;
+    }
+};
+
+// ======================================================================
+// Block-0 prologues of a column pass: small serial work that must follow the
+// previous kernel (its partial sums are complete and visible at the kernel
+// boundary) and needs no launch of its own.
+// ======================================================================
+struct ProNone {
+    static constexpr bool NONE = true;
+    __device__ int nblk() const { return 0; }
+    __device__ void operator()(double *) const {}
+};
+
+// The two halves of closing one gradient-flow iteration.  Partial sums:
+// pA from the i pass (w G |U|^2, w |G U - f_hat|^2, max residual), pC from
+// the row pass of the same field (density).
+// First level (blocks b < kf of the next pass): block b reduces its share of
+// pA and pC into p2[q * kf + b].
+struct ProGradFlowReduce {
+    static constexpr bool NONE = false;
+    const double *pA;
+    long nA;
+    const double *pC;
+    long nC;
+    double *p2;
+    int kf;
+    __device__ int nblk() const { return kf; }
+    __device__ void operator()(double *red) const
+    {
+        const long b = blockIdx.x;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (long i = nA * b / kf + threadIdx.x; i < nA * (b + 1) / kf; i += blockDim.x) {
+            v[0] += pA[i];
+            v[1] += pA[nA + i];
+            v[2] = fmax(v[2], pA[2 * nA + i]);
+        }
+        for (long i = nC * b / kf + threadIdx.x; i < nC * (b + 1) / kf; i += blockDim.x) v[3] += pC[i];
+        block_reduce<4, 4u>(v, red);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) p2[q * kf + b] = v[q];
+        __syncthreads();
+    }
+};
+
+// Second level (block 0 of the pass after that): E = E1 + E2, ||grad F||,
+// residual into hist[3*iter ..], iter += 1, and the stop test of LPsrc.cpp:52.
+struct ProGradFlowFinalize {
+    static constexpr bool NONE = false;
+    const double *p2;
+    int kf;
+    double R;
+    Ctl *ctl;
+    double *hist;
+    __device__ int nblk() const { return 1; }
+    __device__ void operator()(double *red) const
+    {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < kf; b += blockDim.x) {
+            v[0] += p2[b];
+            v[1] += p2[kf + b];
+            v[2] = fmax(v[2], p2[2 * kf + b]);
+            v[3] += p2[3 * kf + b];
+        }
+        block_reduce<4, 4u>(v, red);
+        if (threadIdx.x == 0) {
+            const double E = 0.5 * v[0] + v[3] / R, gn = sqrt(v[1]), res = v[2];
+            const int it = ctl->iter;
+            hist[3L * it + 0] = E;
+            hist[3L * it + 1] = gn;
+            hist[3L * it + 2] = res;
+            ctl->iter = it + 1;
+            if (!(res > ctl->res_tol) || (ctl->gn_tol > 0 && gn <= ctl->gn_tol) || it + 1 >= ctl->max_iter)
+                ctl->stop = 1;
+        }
+        __syncthreads();
     }
 };
 
@@ -274,24 +380,41 @@ __device__ __forceinline__ void write_partials(double (&acc)[NRED > 0 ? NRED : 1
     }
 }
 
+// Twiddle table exp(-2 pi i e / N) into LDS.  The first Stockham stage uses no
+// twiddles and ends in a barrier, so no barrier is needed here.
+template <typename T, int N>
+__device__ __forceinline__ void load_twiddles(C2<T> *stw, const C2<T> *__restrict__ tw)
+{
+    for (int e = threadIdx.x; e < N; e += blockDim.x) stw[e] = tw[e];
+}
+
 // Row (k-axis) pass over npairs pairs of rows.
 //   IN_SPEC:  input rows are half spectra (row stride NHP): c2r (unnormalised)
 //   !IN_SPEC: input rows are real (row stride N)
 //   OUT_SPEC: forward r2c (unnormalised) of op(x) into out (row stride NHP)
-template <typename T, int N, bool IN_SPEC, bool OUT_SPEC, class Op>
+// The block's input rows are contiguous in memory: they are staged through
+// LDS with coalesced loads, each element read once.
+template <typename T, int N, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
 __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
-    k_rows(const void *in, C2<T> *out, int NHP, long npairs,
-           const C2<T> *__restrict__ tw, Op op, double *__restrict__ part, const Ctl *__restrict__ ctl)
+    k_rows(const void *in, C2<T> *out, int NHP, long npairs, const C2<T> *__restrict__ tw, Op op,
+           double *__restrict__ part, const Ctl *ctl, Pro pro = Pro{})
 {
     using G = Geo<T, N>;
-    constexpr int TPP = G::TPP, E = G::E, PP = G::PP;
-    __shared__ C2<T> lds[PP * G::RSTR];
-    __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16];
+    constexpr int TPP = G::TPP, E = G::E, PP = G::PP, NH = N / 2 + 1;
+    constexpr int XCH = PP * G::RSTR, STG = IN_SPEC ? 2 * PP * NH : PP * N;
+    __shared__ C2<T> lds[XCH > STG ? XCH : STG];
+    __shared__ C2<T> stw[N];
+    __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
     if (ctl && ctl->stop) return;
+    if constexpr (!Pro::NONE)
+        if ((int)blockIdx.x < pro.nblk()) pro(red);
+    load_twiddles<T, N>(stw, tw);
     const int t = threadIdx.x % TPP, pp = threadIdx.x / TPP;
     const long pair = (long)blockIdx.x * PP + pp;
     const bool active = pair < npairs;
     const long ra = 2 * pair, rb = ra + 1;
+    const long row0 = 2L * PP * blockIdx.x;
+    const long nrows = 2 * npairs;
     C2<T> *my = lds + pp * G::RSTR;
     C2<T> v[E];
     double acc[Op::NRED > 0 ? Op::NRED : 1];
@@ -300,15 +423,17 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
 
     if constexpr (IN_SPEC) {
         const C2<T> *S = (const C2<T> *)in;
+        for (int idx = threadIdx.x; idx < 2 * PP * NH; idx += blockDim.x) {
+            const int r = idx / NH, k = idx - r * NH;
+            if (row0 + r < nrows) lds[idx] = S[(row0 + r) * NHP + k];
+        }
+        __syncthreads();
+        const C2<T> *la = lds + (2 * pp) * NH, *lb = la + NH;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = t + TPP * m;
             const int q = p <= N / 2 ? p : N - p;
-            C2<T> a = {0, 0}, b = {0, 0};
-            if (active) {
-                a = S[ra * NHP + q];
-                b = S[rb * NHP + q];
-            }
+            C2<T> a = la[q], b = lb[q];
             if (q == 0 || q == N / 2) { // c2r reads only the real part of DC and Nyquist
                 a.y = 0;
                 b.y = 0;
@@ -316,13 +441,23 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
             // Z = Xa + i Xb on 0..N/2, conj(Xa) + i conj(Xb) above
             v[m] = p <= N / 2 ? C2<T>{a.x - b.y, a.y + b.x} : C2<T>{a.x + b.y, b.x - a.y};
         }
-        fft_pencil<T, N, +1, 1>(v, my, tw, t);
+        fft_pencil<T, N, +1, 1>(v, my, stw, t);
     } else {
+        // 16-byte chunks of the block's 2*PP contiguous real rows
+        constexpr int VPC = 16 / sizeof(T);
+        using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
         const T *X = (const T *)in;
+        const V *src = (const V *)(X + row0 * N);
+        V *dstl = (V *)lds;
+        const long nval = (nrows - row0 < 2 * PP ? nrows - row0 : 2 * PP) * (long)N;
+        for (int c = threadIdx.x; c < 2 * PP * N / VPC; c += blockDim.x)
+            if ((long)c * VPC < nval) dstl[c] = src[c];
+        __syncthreads();
+        const T *lr = (const T *)lds + (2 * pp) * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = t + TPP * m;
-            v[m] = active ? C2<T>{X[ra * N + p], X[rb * N + p]} : C2<T>{0, 0};
+            v[m] = C2<T>{lr[p], lr[N + p]};
         }
     }
     if (active) {
@@ -334,7 +469,7 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
         }
     }
     if constexpr (OUT_SPEC) {
-        fft_pencil<T, N, -1, 1>(v, my, tw, t);
+        fft_pencil<T, N, -1, 1>(v, my, stw, t);
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m];
@@ -366,13 +501,15 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
     using G = Geo<T, N>;
     constexpr int TPP = G::TPP, E = G::E, PP = G::PP;
     __shared__ C2<T> lds[PP * G::RSTR];
+    __shared__ C2<T> stw[N];
+    load_twiddles<T, N>(stw, tw);
     const int t = threadIdx.x % TPP, pp = threadIdx.x / TPP;
     const long row = (long)blockIdx.x * PP + pp;
     const bool active = row < nrows;
     C2<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = active ? in[row * N + t + TPP * m] : C2<T>{0, 0};
-    fft_pencil<T, N, DIR, 1>(v, lds + pp * G::RSTR, tw, t);
+    fft_pencil<T, N, DIR, 1>(v, lds + pp * G::RSTR, stw, t);
     if (active)
 #pragma unroll
         for (int m = 0; m < E; ++m) out[row * N + t + TPP * m] = cscale(v[m], (T)sc);
@@ -383,17 +520,21 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
 // Valid columns: col < ncols and (col mod NHP) < NH.  iaxis: the pencil runs
 // along i (frequencies i = e, j = col / NHP, kz = col mod NHP); otherwise
 // along j (i = o, j = e, kz = col).
-template <typename T, int N, bool FWD, bool INV, bool STORE, class SOp>
+template <typename T, int N, bool FWD, bool INV, bool STORE, class SOp, class Pro = ProNone>
 __global__ void __launch_bounds__((Geo<T, N>::COL_THREADS))
     k_cols(const C2<T> *src, C2<T> *dst, long pstride, long ncols, long ostride, int NH, int NHP,
            int iaxis, const C2<T> *__restrict__ tw, SOp op, double *__restrict__ part,
-           const Ctl *__restrict__ ctl)
+           const Ctl *ctl, Pro pro = Pro{})
 {
     using G = Geo<T, N>;
     constexpr int TPP = G::TPP, E = G::E, B = G::B;
     __shared__ C2<T> lds[N * B];
-    __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16];
+    __shared__ C2<T> stw[N];
+    __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
     if (ctl && ctl->stop) return;
+    if constexpr (!Pro::NONE)
+        if ((int)blockIdx.x < pro.nblk()) pro(red);
+    load_twiddles<T, N>(stw, tw);
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const long tpo = (ncols + B - 1) / B;
     const long o = blockIdx.x / tpo;
@@ -402,22 +543,28 @@ __global__ void __launch_bounds__((Geo<T, N>::COL_THREADS))
     const bool active = col < ncols && kz < NH;
     const long off = o * ostride + col;
     C2<T> v[E];
+    C2<T> pre[SOp::PRELOAD ? E : 1];
     double acc[SOp::NRED > 0 ? SOp::NRED : 1];
 #pragma unroll
     for (int q = 0; q < (SOp::NRED > 0 ? SOp::NRED : 1); ++q) acc[q] = 0.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = active ? src[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
-    if constexpr (FWD) fft_pencil<T, N, -1, B>(v, lds + c, tw, t);
+    if constexpr (SOp::PRELOAD) {
+        const C2<T> *ps = op.pre_src();
+#pragma unroll
+        for (int m = 0; m < E; ++m) pre[m] = active ? ps[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
+    }
+    if constexpr (FWD) fft_pencil<T, N, -1, B>(v, lds + c, stw, t);
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int e = t + TPP * m;
             const int i = iaxis ? e : (int)o;
             const int j = iaxis ? (int)(col / NHP) : e;
-            op(v[m], off + (long)e * pstride, i, j, kz, acc);
+            op(v[m], pre[SOp::PRELOAD ? m : 0], off + (long)e * pstride, i, j, kz, acc);
         }
     }
-    if constexpr (INV) fft_pencil<T, N, +1, B>(v, lds + c, tw, t);
+    if constexpr (INV) fft_pencil<T, N, +1, B>(v, lds + c, stw, t);
     if constexpr (STORE) {
         if (active)
 #pragma unroll

@@ -9,32 +9,17 @@
 #include <string>
 #include <vector>
 
-#include "../../include/spm_iqc.h"
-#include "spm_ctx.h"
+#include "capi_internal.h"
 
 using namespace spm;
 
-struct spm_ctx {
-    Ctx c;
-    hipStream_t own = nullptr;
-    double *h_pin = nullptr; // pinned host scratch for scalars read back
-    size_t dev_bytes = 0;
-};
-
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg)
+int spm_fail(int code, const std::string &msg)
 {
     g_err = msg;
     return code;
 }
-
-#define HIPCHK(expr)                                                                      \
-    do {                                                                                  \
-        hipError_t _e = (expr);                                                           \
-        if (_e != hipSuccess)                                                             \
-            return fail(SPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
-    } while (0)
 
 // ======================================================================
 // generic kernels
@@ -56,34 +41,6 @@ __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part,
     if (mx) block_reduce<1, 1u>(v, red);
     else block_reduce<1, 0u>(v, red);
     if (threadIdx.x == 0) out[q] = mx ? v[0] : v[0] * scale;
-}
-
-// Closes one gradient-flow iteration: E = E1 + E2, ||grad F||, residual into
-// hist[3*iter ..], then iter += 1 and the stop test of LPsrc.cpp:52.
-__global__ void __launch_bounds__(256) k_gf_finalize(const double *__restrict__ pA, long nA,
-                                                    const double *__restrict__ pC, long nC, double R, Ctl *ctl,
-                                                    double *__restrict__ hist)
-{
-    __shared__ double red[4 * 16];
-    if (ctl->stop) return;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (long b = threadIdx.x; b < nA; b += blockDim.x) {
-        v[0] += pA[b];
-        v[1] += pA[nA + b];
-        v[2] = fmax(v[2], pA[2 * nA + b]);
-    }
-    for (long b = threadIdx.x; b < nC; b += blockDim.x) v[3] += pC[b];
-    block_reduce<4, 4u>(v, red);
-    if (threadIdx.x == 0) {
-        const double E = 0.5 * v[0] + v[3] / R, gn = sqrt(v[1]), res = v[2];
-        const int it = ctl->iter;
-        hist[3L * it + 0] = E;
-        hist[3L * it + 1] = gn;
-        hist[3L * it + 2] = res;
-        ctl->iter = it + 1;
-        if (!(res > ctl->res_tol) || (ctl->gn_tol > 0 && gn <= ctl->gn_tol) || it + 1 >= ctl->max_iter)
-            ctl->stop = 1;
-    }
 }
 
 __global__ void k_ctl_init(Ctl *ctl, int max_iter, double res_tol, double gn_tol)
@@ -137,12 +94,6 @@ void launch_reduce(Ctx &c, const double *part, long nblk, int nval, unsigned max
     hipLaunchKernelGGL(k_reduce, dim3(nval), dim3(256), 0, c.stream, part, nblk, maxmask, scale, out);
 }
 
-void launch_gf_finalize(Ctx &c, const double *pA, long nA, const double *pC, long nC)
-{
-    hipLaunchKernelGGL(k_gf_finalize, dim3(1), dim3(256), 0, c.stream, pA, nA, pC, nC, (double)c.R, c.ctl,
-                       c.hist);
-}
-
 GridIface *make_grid_n4(int), *make_grid_n8(int), *make_grid_n12(int), *make_grid_n16(int),
     *make_grid_n24(int), *make_grid_n32(int), *make_grid_n48(int), *make_grid_n64(int), *make_grid_n96(int),
     *make_grid_n128(int), *make_grid_n192(int), *make_grid_n256(int);
@@ -184,11 +135,11 @@ extern "C" double spm_get_GK(double k2)
     return exp(-0.5 * k2 / s2) * (d0 + d2 * k2 + d4 * k4 + d6 * k6 + d8 * k8) + 0.08;
 }
 
-static int dev_alloc(spm_ctx *h, void **p, size_t bytes)
+int spm_dev_alloc(spm_ctx *h, void **p, size_t bytes)
 {
     if (hipMalloc(p, bytes) != hipSuccess) {
         *p = nullptr;
-        return fail(SPM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return spm_fail(SPM_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
     h->dev_bytes += bytes;
     return SPM_OK;
@@ -197,10 +148,12 @@ static int dev_alloc(spm_ctx *h, void **p, size_t bytes)
 static void free_all(spm_ctx *h)
 {
     Ctx &c = h->c;
-    void *ps[] = {c.tw, c.Gtab, c.Itab, c.W, c.U, c.part, c.scal, c.ctl, c.hist};
+    void *ps[] = {c.tw, c.Gtab, c.Itab, c.Btab, c.GDtab, c.W, c.U, c.part, c.scal, c.ctl, c.hist, c.counter};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->h_pin) (void)hipHostFree(h->h_pin);
+    for (void *p : {h->ws, (void *)h->bscr})
+        if (p) (void)hipFree(p);
     if (h->own) (void)hipStreamDestroy(h->own);
     delete c.g;
 }
@@ -226,7 +179,7 @@ static int ensure_hist(spm_ctx *h, long n)
     }
     long cap = 1024;
     while (cap < n) cap *= 2;
-    int rc = dev_alloc(h, (void **)&c.hist, sizeof(double) * 3 * cap);
+    int rc = spm_dev_alloc(h, (void **)&c.hist, sizeof(double) * 3 * cap);
     if (rc) return rc;
     c.hist_cap = cap;
     return SPM_OK;
@@ -241,13 +194,13 @@ const char *spm_last_error(void) { return g_err.c_str(); }
 
 int spm_create(const spm_params *p, spm_ctx **out)
 {
-    if (!p || !out) return fail(SPM_EINVAL, "spm_create: null argument");
+    if (!p || !out) return spm_fail(SPM_EINVAL, "spm_create: null argument");
     *out = nullptr;
-    if (!supported_N(p->N)) return fail(SPM_ENOTSUP, "spm_create: unsupported N " + std::to_string(p->N));
+    if (!supported_N(p->N)) return spm_fail(SPM_ENOTSUP, "spm_create: unsupported N " + std::to_string(p->N));
     if (p->precision != SPM_FP64 && p->precision != SPM_FP32)
-        return fail(SPM_EINVAL, "spm_create: precision must be SPM_FP64 or SPM_FP32");
+        return spm_fail(SPM_EINVAL, "spm_create: precision must be SPM_FP64 or SPM_FP32");
     if (!(p->L > 0) || !(p->kscale > 0) || !std::isfinite(p->Tau) || !std::isfinite(p->Gamma) || !(p->dt > 0))
-        return fail(SPM_EINVAL, "spm_create: L, kscale, dt must be > 0 and Tau, Gamma finite");
+        return spm_fail(SPM_EINVAL, "spm_create: L, kscale, dt must be > 0 and Tau, Gamma finite");
     spm_ctx *h = new spm_ctx();
     Ctx &c = h->c;
     c.N = p->N;
@@ -266,7 +219,7 @@ int spm_create(const spm_params *p, spm_ctx **out)
     c.g = make_grid(c.N, c.prec);
     if (!c.g) {
         delete h;
-        return fail(SPM_ENOTSUP, "spm_create: no kernels for this N");
+        return spm_fail(SPM_ENOTSUP, "spm_create: no kernels for this N");
     }
     c.g->geometry(c.nb_rows, c.nb_j, c.nb_i, c.NHP);
     int rc = SPM_OK;
@@ -276,7 +229,7 @@ int spm_create(const spm_params *p, spm_ctx **out)
         return code;
     };
     if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(SPM_EHIP, "spm_create: hipStreamCreate failed (no GPU?)"));
+        return bail(spm_fail(SPM_EHIP, "spm_create: hipStreamCreate failed (no GPU?)"));
     c.stream = h->own;
     // twiddles exp(-2 pi i e / N)
     {
@@ -289,43 +242,57 @@ int spm_create(const spm_params *p, spm_ctx **out)
             twf[2 * e] = (float)cosl(a);
             twf[2 * e + 1] = (float)(-sinl(a));
         }
-        if ((rc = dev_alloc(h, &c.tw, 2 * c.N * c.esz))) return bail(rc);
+        if ((rc = spm_dev_alloc(h, &c.tw, 2 * c.N * c.esz))) return bail(rc);
         if (hipMemcpy(c.tw, c.prec == SPM_FP64 ? (void *)twd.data() : (void *)twf.data(), 2 * c.N * c.esz,
                       hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(SPM_EHIP, "spm_create: twiddle upload failed"));
+            return bail(spm_fail(SPM_EHIP, "spm_create: twiddle upload failed"));
     }
     // G and preconditioner tables over |n|^2 (get_ksquare_local, get_epmckt2_local,
     // get_ikt2_local; Initialization.cpp:123-174)
     {
         const int h2 = c.N / 2;
         c.ntab = 3 * h2 * h2 + 1;
-        std::vector<double> Gt(c.ntab), It(c.ntab);
+        std::vector<double> Gt(c.ntab), It(c.ntab), Bt(c.ntab), GDd(2 * c.ntab);
+        std::vector<float> GDf(2 * c.ntab);
         const double P = c.kscale / c.L;
         for (int n2 = 0; n2 < c.ntab; ++n2) {
             const double k2 = (double)((long double)P * (long double)P * (long double)n2);
             Gt[n2] = spm_get_GK(k2);
             It[n2] = 1.0 / (Gt[n2] + std::fabs(c.Tau) + 0.02);
+            Bt[n2] = 1.0 / (1 + 1.0 * Gt[n2]);
+            GDd[2 * n2] = Gt[n2];
+            GDd[2 * n2 + 1] = 1.0 / (1 + c.dt * Gt[n2]);
+            GDf[2 * n2] = (float)GDd[2 * n2];
+            GDf[2 * n2 + 1] = (float)GDd[2 * n2 + 1];
         }
-        if ((rc = dev_alloc(h, (void **)&c.Gtab, sizeof(double) * c.ntab))) return bail(rc);
-        if ((rc = dev_alloc(h, (void **)&c.Itab, sizeof(double) * c.ntab))) return bail(rc);
+        if ((rc = spm_dev_alloc(h, (void **)&c.Gtab, sizeof(double) * c.ntab))) return bail(rc);
+        if ((rc = spm_dev_alloc(h, (void **)&c.Itab, sizeof(double) * c.ntab))) return bail(rc);
+        if ((rc = spm_dev_alloc(h, &c.GDtab, 2 * c.esz * c.ntab))) return bail(rc);
+        if ((rc = spm_dev_alloc(h, (void **)&c.Btab, sizeof(double) * c.ntab))) return bail(rc);
         if (hipMemcpy(c.Gtab, Gt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c.Itab, It.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(SPM_EHIP, "spm_create: table upload failed"));
+            hipMemcpy(c.Itab, It.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c.Btab, Bt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c.GDtab, c.prec == SPM_FP64 ? (const void *)GDd.data() : (const void *)GDf.data(),
+                      2 * c.esz * c.ntab, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(spm_fail(SPM_EHIP, "spm_create: table upload failed"));
     }
     const size_t spec = (size_t)c.C * 2 * c.esz;
-    if ((rc = dev_alloc(h, &c.W, spec))) return bail(rc);
-    if ((rc = dev_alloc(h, &c.U, spec))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, &c.W, spec))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, &c.U, spec))) return bail(rc);
     long mb = std::max(std::max(c.nb_rows, c.nb_j), std::max(c.nb_i, (long)DOT_BLOCKS));
     c.part_slot = 4 * mb;
     c.nslots = 8;
-    if ((rc = dev_alloc(h, (void **)&c.part, sizeof(double) * c.part_slot * c.nslots))) return bail(rc);
-    if ((rc = dev_alloc(h, (void **)&c.scal, sizeof(double) * SC_COUNT))) return bail(rc);
-    if ((rc = dev_alloc(h, (void **)&c.ctl, sizeof(Ctl)))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, (void **)&c.part, sizeof(double) * c.part_slot * c.nslots))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, (void **)&c.scal, sizeof(double) * SC_COUNT))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, (void **)&c.ctl, sizeof(Ctl)))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, (void **)&c.counter, 64 * sizeof(unsigned)))) return bail(rc);
+    if (hipMemset(c.counter, 0, 64 * sizeof(unsigned)) != hipSuccess)
+        return bail(spm_fail(SPM_EHIP, "spm_create: memset"));
     if (hipHostMalloc((void **)&h->h_pin, 4096, hipHostMallocDefault) != hipSuccess)
-        return bail(fail(SPM_ENOMEM, "spm_create: pinned scratch"));
+        return bail(spm_fail(SPM_ENOMEM, "spm_create: pinned scratch"));
     if ((rc = ensure_hist(h, 1024))) return bail(rc);
     if (hipMemset(c.scal, 0, sizeof(double) * SC_COUNT) != hipSuccess)
-        return bail(fail(SPM_EHIP, "spm_create: memset"));
+        return bail(spm_fail(SPM_EHIP, "spm_create: memset"));
     *out = h;
     return SPM_OK;
 }
@@ -341,14 +308,14 @@ int spm_destroy(spm_ctx *h)
 
 int spm_set_stream(spm_ctx *h, void *s)
 {
-    if (!h) return fail(SPM_EINVAL, "null context");
+    if (!h) return spm_fail(SPM_EINVAL, "null context");
     h->c.stream = (hipStream_t)s; // NULL: the legacy default stream
     return SPM_OK;
 }
 
 int spm_get_layout(const spm_ctx *h, spm_layout *o)
 {
-    if (!h || !o) return fail(SPM_EINVAL, "null argument");
+    if (!h || !o) return spm_fail(SPM_EINVAL, "null argument");
     const Ctx &c = h->c;
     o->N = c.N;
     o->nh = c.NH;
@@ -362,51 +329,53 @@ int spm_get_layout(const spm_ctx *h, spm_layout *o)
     return SPM_OK;
 }
 
-#define CHECK_CTX(h)                                                                     \
-    if (!(h)) return fail(SPM_EINVAL, "null context");                                  \
-    Ctx &c = (h)->c;                                                                    \
-    (void)c
 
-static int launch_status(const char *what)
+} // extern "C"
+
+int spm_launch_status(const char *what)
 {
     const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(SPM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    if (e != hipSuccess) return spm_fail(SPM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
     return SPM_OK;
 }
+
+extern "C" {
 
 int spm_my_fft(spm_ctx *h, const void *x_R, void *x_F)
 {
     CHECK_CTX(h);
-    if (!x_R || !x_F) return fail(SPM_EINVAL, "spm_my_fft: null field");
+    if (!x_R || !x_F) return spm_fail(SPM_EINVAL, "spm_my_fft: null field");
     c.g->c2c(c, x_R, x_F, -1);
-    return launch_status("spm_my_fft");
+    return spm_launch_status("spm_my_fft");
 }
 
 int spm_my_ifft(spm_ctx *h, const void *x_F, void *x_R)
 {
     CHECK_CTX(h);
-    if (!x_R || !x_F) return fail(SPM_EINVAL, "spm_my_ifft: null field");
+    if (!x_R || !x_F) return spm_fail(SPM_EINVAL, "spm_my_ifft: null field");
     c.g->c2c(c, x_F, x_R, +1);
-    return launch_status("spm_my_ifft");
+    return spm_launch_status("spm_my_ifft");
 }
 
 int spm_r2c(spm_ctx *h, const void *x, void *S)
 {
     CHECK_CTX(h);
-    if (!x || !S) return fail(SPM_EINVAL, "spm_r2c: null field");
+    if (!x || !S) return spm_fail(SPM_EINVAL, "spm_r2c: null field");
     c.g->r2c(c, x, S);
-    return launch_status("spm_r2c");
+    return spm_launch_status("spm_r2c");
 }
 
 int spm_c2r(spm_ctx *h, const void *S, void *x)
 {
     CHECK_CTX(h);
-    if (!x || !S) return fail(SPM_EINVAL, "spm_c2r: null field");
+    if (!x || !S) return spm_fail(SPM_EINVAL, "spm_c2r: null field");
     c.g->c2r(c, S, x);
-    return launch_status("spm_c2r");
+    return spm_launch_status("spm_c2r");
 }
 
-static int read_scalars(spm_ctx *h, const double *dev, int n, double *out)
+} // extern "C"
+
+int spm_read_scalars(spm_ctx *h, const double *dev, int n, double *out)
 {
     Ctx &c = h->c;
     HIPCHK(hipMemcpyAsync(h->h_pin, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
@@ -415,15 +384,17 @@ static int read_scalars(spm_ctx *h, const double *dev, int n, double *out)
     return SPM_OK;
 }
 
+extern "C" {
+
 int spm_energy(spm_ctx *h, const void *phi, double *E)
 {
     CHECK_CTX(h);
-    if (!phi || !E) return fail(SPM_EINVAL, "spm_energy: null argument");
+    if (!phi || !E) return spm_fail(SPM_EINVAL, "spm_energy: null argument");
     c.g->energy(c, phi);
-    int rc = launch_status("spm_energy");
+    int rc = spm_launch_status("spm_energy");
     if (rc) return rc;
     double v[2];
-    if ((rc = read_scalars(h, c.scal + SC_E1, 2, v))) return rc;
+    if ((rc = spm_read_scalars(h, c.scal + SC_E1, 2, v))) return rc;
     *E = 0.5 * v[0] + v[1]; // E1 + E2 (LPsrc.cpp:35-36)
     return SPM_OK;
 }
@@ -431,13 +402,13 @@ int spm_energy(spm_ctx *h, const void *phi, double *E)
 int spm_ngrad(spm_ctx *h, const void *phi, void *grad, double *nrm)
 {
     CHECK_CTX(h);
-    if (!phi || !grad) return fail(SPM_EINVAL, "spm_ngrad: null field");
-    if (phi == grad) return fail(SPM_EINVAL, "spm_ngrad: phi and grad must differ");
+    if (!phi || !grad) return spm_fail(SPM_EINVAL, "spm_ngrad: null field");
+    if (phi == grad) return spm_fail(SPM_EINVAL, "spm_ngrad: phi and grad must differ");
     c.g->ngrad(c, phi, grad);
-    int rc = launch_status("spm_ngrad");
+    int rc = spm_launch_status("spm_ngrad");
     if (rc || !nrm) return rc;
     double g2;
-    if ((rc = read_scalars(h, c.scal + SC_GN2, 1, &g2))) return rc;
+    if ((rc = spm_read_scalars(h, c.scal + SC_GN2, 1, &g2))) return rc;
     *nrm = std::sqrt(g2);
     return SPM_OK;
 }
@@ -445,26 +416,26 @@ int spm_ngrad(spm_ctx *h, const void *phi, void *grad, double *nrm)
 int spm_hv(spm_ctx *h, const void *phi, const void *v, void *hv)
 {
     CHECK_CTX(h);
-    if (!phi || !v || !hv) return fail(SPM_EINVAL, "spm_hv: null field");
-    if (hv == v || hv == phi) return fail(SPM_EINVAL, "spm_hv: hv must not alias phi or v");
+    if (!phi || !v || !hv) return spm_fail(SPM_EINVAL, "spm_hv: null field");
+    if (hv == v || hv == phi) return spm_fail(SPM_EINVAL, "spm_hv: hv must not alias phi or v");
     c.g->hv(c, phi, v, hv);
-    return launch_status("spm_hv");
+    return spm_launch_status("spm_hv");
 }
 
 int spm_precond(spm_ctx *h, void *w)
 {
     CHECK_CTX(h);
-    if (!w) return fail(SPM_EINVAL, "spm_precond: null field");
+    if (!w) return spm_fail(SPM_EINVAL, "spm_precond: null field");
     c.g->precond(c, w);
-    return launch_status("spm_precond");
+    return spm_launch_status("spm_precond");
 }
 
 int spm_gradflow(spm_ctx *h, void *phi, int max_iter, double res_tol, double gn_tol, int check_every,
                  double *hist, int *iters)
 {
     CHECK_CTX(h);
-    if (!phi) return fail(SPM_EINVAL, "spm_gradflow: null field");
-    if (max_iter < 0 || check_every < 1) return fail(SPM_EINVAL, "spm_gradflow: max_iter >= 0, check_every >= 1");
+    if (!phi) return spm_fail(SPM_EINVAL, "spm_gradflow: null field");
+    if (max_iter < 0 || check_every < 1) return spm_fail(SPM_EINVAL, "spm_gradflow: max_iter >= 0, check_every >= 1");
     int rc = ensure_hist(h, max_iter > 0 ? max_iter : 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, c.stream, c.ctl, max_iter, res_tol, gn_tol);
@@ -475,7 +446,7 @@ int spm_gradflow(spm_ctx *h, void *phi, int max_iter, double res_tol, double gn_
             const long n = std::min<long>(check_every, max_iter - queued);
             for (long s = 0; s < n; ++s) c.g->gf_step(c);
             queued += n;
-            if ((rc = launch_status("spm_gradflow"))) return rc;
+            if ((rc = spm_launch_status("spm_gradflow"))) return rc;
             HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
             HIPCHK(hipStreamSynchronize(c.stream));
             if (((Ctl *)h->h_pin)->stop) break;
@@ -490,27 +461,27 @@ int spm_gradflow(spm_ctx *h, void *phi, int max_iter, double res_tol, double gn_
         HIPCHK(hipMemcpyAsync(hist, c.hist, sizeof(double) * 3 * it, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
     }
-    return launch_status("spm_gradflow");
+    return spm_launch_status("spm_gradflow");
 }
 
 int spm_gf_begin(spm_ctx *h, const void *phi, int max_iter, double res_tol, double gn_tol)
 {
     CHECK_CTX(h);
-    if (!phi) return fail(SPM_EINVAL, "spm_gf_begin: null field");
-    if (max_iter < 0) return fail(SPM_EINVAL, "spm_gf_begin: max_iter >= 0");
+    if (!phi) return spm_fail(SPM_EINVAL, "spm_gf_begin: null field");
+    if (max_iter < 0) return spm_fail(SPM_EINVAL, "spm_gf_begin: max_iter >= 0");
     int rc = ensure_hist(h, max_iter > 0 ? max_iter : 1);
     if (rc) return rc;
     hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, c.stream, c.ctl, max_iter, res_tol, gn_tol);
     if (max_iter > 0) c.g->gf_init(c, phi);
-    return launch_status("spm_gf_begin");
+    return spm_launch_status("spm_gf_begin");
 }
 
 int spm_gf_steps(spm_ctx *h, int nsteps)
 {
     CHECK_CTX(h);
-    if (nsteps < 0) return fail(SPM_EINVAL, "spm_gf_steps: nsteps >= 0");
+    if (nsteps < 0) return spm_fail(SPM_EINVAL, "spm_gf_steps: nsteps >= 0");
     for (int s = 0; s < nsteps; ++s) c.g->gf_step(c);
-    return launch_status("spm_gf_steps");
+    return spm_launch_status("spm_gf_steps");
 }
 
 int spm_gf_end(spm_ctx *h, void *phi, double *hist, int *iters)
@@ -525,13 +496,13 @@ int spm_gf_end(spm_ctx *h, void *phi, double *hist, int *iters)
         HIPCHK(hipMemcpyAsync(hist, c.hist, sizeof(double) * 3 * it, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
     }
-    return launch_status("spm_gf_end");
+    return spm_launch_status("spm_gf_end");
 }
 
 int spm_inpfp(spm_ctx *h, const void *p, const void *q, double *out)
 {
     CHECK_CTX(h);
-    if (!p || !q || !out) return fail(SPM_EINVAL, "spm_inpfp: null argument");
+    if (!p || !q || !out) return spm_fail(SPM_EINVAL, "spm_inpfp: null argument");
     double *part = c.slot(7);
     if (c.prec == SPM_FP64)
         hipLaunchKernelGGL(k_dot<double>, dim3(DOT_BLOCKS), dim3(256), 0, c.stream, (const double *)p,
@@ -540,9 +511,9 @@ int spm_inpfp(spm_ctx *h, const void *p, const void *q, double *out)
         hipLaunchKernelGGL(k_dot<float>, dim3(DOT_BLOCKS), dim3(256), 0, c.stream, (const float *)p,
                            (const float *)q, c.R, part);
     launch_reduce(c, part, DOT_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_TMP0);
-    int rc = launch_status("spm_inpfp");
+    int rc = spm_launch_status("spm_inpfp");
     if (rc) return rc;
-    return read_scalars(h, c.scal + SC_TMP0, 1, out);
+    return spm_read_scalars(h, c.scal + SC_TMP0, 1, out);
 }
 
 int spm_nrmp(spm_ctx *h, const void *p, double *out)
@@ -555,7 +526,7 @@ int spm_nrmp(spm_ctx *h, const void *p, double *out)
 int spm_lin2(spm_ctx *h, void *r, double a1, const void *F1, double a2, const void *F2)
 {
     CHECK_CTX(h);
-    if (!r || !F1 || !F2) return fail(SPM_EINVAL, "spm_lin2: null field");
+    if (!r || !F1 || !F2) return spm_fail(SPM_EINVAL, "spm_lin2: null field");
     const int nb = 2048;
     if (c.prec == SPM_FP64)
         hipLaunchKernelGGL(k_lin2<double>, dim3(nb), dim3(256), 0, c.stream, (double *)r, a1, (const double *)F1,
@@ -563,33 +534,33 @@ int spm_lin2(spm_ctx *h, void *r, double a1, const void *F1, double a2, const vo
     else
         hipLaunchKernelGGL(k_lin2<float>, dim3(nb), dim3(256), 0, c.stream, (float *)r, a1, (const float *)F1, a2,
                            (const float *)F2, c.R);
-    return launch_status("spm_lin2");
+    return spm_launch_status("spm_lin2");
 }
 
 int spm_cplx_to_real(spm_ctx *h, const void *z, void *x)
 {
     CHECK_CTX(h);
-    if (!z || !x) return fail(SPM_EINVAL, "null field");
+    if (!z || !x) return spm_fail(SPM_EINVAL, "null field");
     if (c.prec == SPM_FP64)
         hipLaunchKernelGGL(k_cplx_to_real<double>, dim3(2048), dim3(256), 0, c.stream, (const C2<double> *)z,
                            (double *)x, c.R);
     else
         hipLaunchKernelGGL(k_cplx_to_real<float>, dim3(2048), dim3(256), 0, c.stream, (const C2<float> *)z,
                            (float *)x, c.R);
-    return launch_status("spm_cplx_to_real");
+    return spm_launch_status("spm_cplx_to_real");
 }
 
 int spm_real_to_cplx(spm_ctx *h, const void *x, void *z)
 {
     CHECK_CTX(h);
-    if (!z || !x) return fail(SPM_EINVAL, "null field");
+    if (!z || !x) return spm_fail(SPM_EINVAL, "null field");
     if (c.prec == SPM_FP64)
         hipLaunchKernelGGL(k_real_to_cplx<double>, dim3(2048), dim3(256), 0, c.stream, (const double *)x,
                            (C2<double> *)z, c.R);
     else
         hipLaunchKernelGGL(k_real_to_cplx<float>, dim3(2048), dim3(256), 0, c.stream, (const float *)x,
                            (C2<float> *)z, c.R);
-    return launch_status("spm_real_to_cplx");
+    return spm_launch_status("spm_real_to_cplx");
 }
 
 } // extern "C"

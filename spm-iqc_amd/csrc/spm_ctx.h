@@ -16,7 +16,8 @@ namespace spm {
 struct GridIface;
 
 enum ScalarSlot { // device scalars (ctx.scal)
-    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_COUNT = 64
+    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_FV, SC_TV2, SC_RES0, SC_RES1,
+    SC_COUNT = 64
 };
 
 struct Ctx {
@@ -31,6 +32,8 @@ struct Ctx {
     void *tw = nullptr;      // exp(-2 pi i e / N), e < N, in the context precision
     double *Gtab = nullptr;  // get_GK(kscale^2 n2 / L^2), n2 = |n|^2 = 0..3(N/2)^2
     double *Itab = nullptr;  // 1 / (G + |Tau| + 0.02)  (get_ikt2_local)
+    double *Btab = nullptr;  // 1 / (1 + G): the SPM force denominator with beta = 1 (SPM.cpp:84-85)
+    void *GDtab = nullptr;   // (G, 1 / (1 + dt G)) pairs in the context precision (LPsrc.cpp:67-68)
     int ntab = 0;
     // work buffers
     void *W = nullptr;       // spectrum-sized work array
@@ -40,6 +43,7 @@ struct Ctx {
     int nslots = 0;
     double *scal = nullptr;  // SC_COUNT device scalars
     Ctl *ctl = nullptr;
+    unsigned *counter = nullptr; // arrival counters of last-block reductions (kept at 0 between launches)
     double *hist = nullptr;  // 3 doubles per iteration: E, ||grad F||, res
     long hist_cap = 0;
     // block counts of the passes over one field
@@ -64,6 +68,9 @@ struct GridIface {
     virtual void ngrad(Ctx &c, const void *phi, void *grad) = 0; // scal[SC_GN2] = nrmp^2
     virtual void hv(Ctx &c, const void *phi, const void *v, void *hv) = 0;
     virtual void precond(Ctx &c, void *w) = 0;
+    // x <- IFFT(FFT(x + nln(x) - coef * (*s) * v) / N^3 / (1 + G)), DC = 0
+    // (one force step of SPM.cpp:34-99 / 103-168; s a device scalar)
+    virtual void force(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;
     virtual void gf_init(Ctx &c, const void *phi) = 0;
     virtual void gf_step(Ctx &c) = 0;
 };
@@ -72,6 +79,5 @@ GridIface *make_grid(int N, int prec);
 
 // generic helpers (spm_capi.hip)
 void launch_reduce(Ctx &c, const double *part, long nblk, int nval, unsigned maxmask, double scale, double *out);
-void launch_gf_finalize(Ctx &c, const double *pA, long nA, const double *pC, long nC);
 
 } // namespace spm

@@ -1,0 +1,188 @@
+"""GPU parity of LOBPCG (LOBPCG.cpp:10-487) and SPM (SPM.cpp:15-263) against
+the CPU FP64 oracle, plus closed-form and brute-force pins.
+
+Eigenvalues are compared in absolute terms at 1e-9 (north_star); eigenvectors
+only through their span, since degenerate eigenspaces admit any basis."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+
+    assert _t.cuda.is_available()
+    return _t
+
+
+def _start(k, N, seed):
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, (k, N, N, N))
+
+
+def _relaxed(po, seeds, N, steps, seed=0):
+    o = po.Oracle(N)
+    phi = seeds.gaussian(N, 0.05, seed)
+    uR = phi.astype(np.complex128)
+    uF = o.my_fft(uR)
+    o.get_min_u_F(uR, uF, steps, 1e-16)
+    return uR.real.copy()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("ma,mb", [(1, 1), (3, 5), (16, 16), (17, 9), (24, 24), (32, 32)])
+def test_gram_matrix_cores(spm, torch, prec, ma, mb):
+    """The MFMA Gram equals the FP64 inner products inpfp of every pair."""
+    N = 16
+    rng = np.random.default_rng(ma * 100 + mb)
+    X = rng.standard_normal((ma, N, N, N))
+    Y = rng.standard_normal((mb, N, N, N))
+    with spm.Grid(N, prec) as g:
+        Xt = torch.as_tensor(X).to(g.dtype).cuda()
+        Yt = torch.as_tensor(Y).to(g.dtype).cuda()
+        out = g.gram(Xt, Yt)
+        Xs, Ys = _np(Xt).astype(np.float64), _np(Yt).astype(np.float64)
+    ref = np.einsum("aijk,bijk->ab", Xs, Ys) / N ** 3
+    assert np.abs(out - ref).max() < 1e-13
+
+
+def test_lobpcg_init_matches_oracle(spm, po, torch):
+    """hiosd_lobpcg_initialization: mean removal + in-order orthonormalisation."""
+    N, k = 16, 4
+    st = _start(k, N, 1)
+    V_ref = po.Oracle(N).lobpcg_initialization(st).real
+    with spm.Grid(N, 0) as g:
+        V = _np(g.lobpcg_init(torch.as_tensor(st).cuda()))
+    assert np.abs(V - V_ref).max() < 1e-12
+    G = np.einsum("aijk,bijk->ab", V, V) / N ** 3
+    assert np.abs(G - np.eye(k)).max() < 1e-13
+
+
+def _closed_form_phi0(N, L=26.0, kscale=1.0, Tau=-0.0061):
+    i = np.arange(N)
+    K = np.where(i < N // 2, i, i - N)
+    n2 = K[:, None, None] ** 2 + K[None, :, None] ** 2 + K[None, None, :] ** 2
+    from pyoracle import get_GK
+
+    G = np.vectorize(get_GK)((kscale / L) ** 2 * n2.astype(float))
+    ev = np.sort((G - Tau)[n2 > 0])
+    return ev
+
+
+@pytest.mark.parametrize("prec", [0])
+def test_lobpcg_phi0_closed_form(spm, po, torch, prec):
+    """At phi = 0 the Hessian is diagonal in Fourier space: eigenvalues
+    G(k) - Tau over k != 0 (mean-free fields)."""
+    N, k = 16, 4
+    ev = _closed_form_phi0(N)
+    st = _start(k, N, 2)
+    with spm.Grid(N, prec) as g:
+        phi = torch.zeros((N,) * 3, dtype=g.dtype, device="cuda")
+        V = g.lobpcg_init(torch.as_tensor(st).to(g.dtype).cuda())
+        ritz = g.lobpcg(phi, V, maxiter=60)
+    assert np.abs(ritz - ev[:k]).max() < 1e-9
+
+
+def _dense_hessian(po, phi, N, Tau=-0.0061, Gamma=0.30):
+    """Dense Hessian of F restricted to mean-free fields (brute force)."""
+    o = po.Oracle(N)
+    _, G, _ = o.tables()
+    R = N ** 3
+    E = np.eye(R).reshape(R, N, N, N)
+    conv = np.fft.ifftn(G[None] * np.fft.fftn(E, axes=(1, 2, 3)), axes=(1, 2, 3)).real.reshape(R, R)
+    c = (-Tau - 2 * Gamma * phi + 3 * phi * phi).ravel()
+    A = conv + np.diag(c)
+    A = 0.5 * (A + A.T)
+    # orthonormal basis of the mean-free subspace
+    Q, _ = np.linalg.qr(np.eye(R)[:, 1:] - 1.0 / R)
+    return np.linalg.eigvalsh(Q.T @ A @ Q)
+
+
+def test_lobpcg_dense_bruteforce(spm, po, seeds, torch):
+    """LOBPCG on a non-trivial field at N = 8 against dense eigvalsh, and the
+    oracle against the same."""
+    N, k = 8, 4
+    phi = seeds.gaussian(N, 0.3, 5)
+    ev = _dense_hessian(po, phi, N)
+    st = _start(k, N, 3)
+    o = po.Oracle(N)
+    Vo = o.lobpcg_initialization(st)
+    ritz_o = None
+    for _ in range(12):
+        ritz_o = o.lobpcg_inip(phi + 0j, Vo, 21)
+    assert np.abs(ritz_o - ev[:k]).max() < 1e-9
+    with spm.Grid(N, 0) as g:
+        pt = torch.as_tensor(phi).cuda()
+        V = g.lobpcg_init(torch.as_tensor(st).cuda())
+        for _ in range(12):
+            ritz = g.lobpcg(pt, V, 21)
+    assert np.abs(ritz - ev[:k]).max() < 1e-9
+
+
+def test_lobpcg_matches_oracle_fixed_iterations(spm, po, seeds, torch):
+    """hiosd_lobpcg_inip from the same start: Ritz values after the
+    reference's 21 iterations, and the spanned subspace.  The field has a
+    separated low spectrum (a relaxed disordered state instead has a ~24-fold
+    near-degenerate cluster, inside which the selected Ritz vectors depend on
+    roundoff; that case is pinned by the phi = 0 closed form)."""
+    N, k = 16, 4
+    phi = seeds.gaussian(N, 0.3, 11)
+    st = _start(k, N, 4)
+    o = po.Oracle(N)
+    Vo = o.lobpcg_initialization(st)
+    ritz_o = o.lobpcg_inip(phi + 0j, Vo, 21)
+    with spm.Grid(N, 0) as g:
+        V = g.lobpcg_init(torch.as_tensor(st).cuda())
+        ritz = g.lobpcg(torch.as_tensor(phi).cuda(), V, 21)
+        V = _np(V)
+    assert np.abs(ritz - ritz_o).max() < 1e-9
+    A = V.reshape(k, -1).T / N ** 1.5
+    B = Vo.real.reshape(k, -1).T / N ** 1.5
+    s = np.linalg.svd(A.T @ B, compute_uv=False)
+    assert s.min() > 1 - 1e-8  # principal angles ~ 0
+
+
+def test_spm_forces_match_oracle(spm, po, seeds, torch):
+    """get_Tv, Natural_Forces, MinMax_Forces (SPM.cpp:15-168)."""
+    N = 32
+    p0 = seeds.icosahedral_nucleus(N, seed=1) if N >= 64 else seeds.gaussian(N, 0.05, 7)
+    p1 = p0 + 0.01 * seeds.gaussian(N, 1.0, 8)
+    o = po.Oracle(N)
+    tv_ref = o.get_Tv(p0 + 0j, p1 + 0j)
+    x_nat = (p0 + 0j).copy()
+    o.Natural_Forces(x_nat, tv_ref)
+    x_mm = (p1 + 0j).copy()
+    o.MinMax_Forces(x_mm, tv_ref)
+    with spm.Grid(N, 0) as g:
+        t0, t1 = torch.as_tensor(p0).cuda(), torch.as_tensor(p1).cuda()
+        tv = g.get_tv(t0, t1)
+        assert np.abs(_np(tv) - tv_ref.real).max() < 1e-12
+        a = t0.clone()
+        g.natural_force(a, tv)
+        b = t1.clone()
+        g.minmax_force(b, tv)
+        assert np.abs(_np(a) - x_nat.real).max() < 1e-12
+        assert np.abs(_np(b) - x_mm.real).max() < 1e-12
+
+
+def test_spring_pairing_matches_oracle(spm, po, seeds, torch):
+    """Spring_Pairing (SPM.cpp:170-263): replicas and residuals per iteration."""
+    N, iters = 16, 12
+    p0 = _relaxed(po, seeds, N, 50, 3)
+    p1 = p0 + 0.001 * seeds.gaussian(N, 1.0, 9)
+    o = po.Oracle(N)
+    P0, P1 = (p0 + 0j).copy(), (p1 + 0j).copy()
+    it_o, res_o = o.Spring_Pairing(P0, P1, iters, 1e-30)
+    with spm.Grid(N, 0) as g:
+        t0, t1 = torch.as_tensor(p0).cuda(), torch.as_tensor(p1).cuda()
+        it, res, hist = g.spring_pairing(t0, t1, iters, 1e-30)
+        a, b = _np(t0), _np(t1)
+    assert it == it_o == iters
+    assert abs(res - res_o) <= 1e-10 * res_o
+    assert np.abs(a - P0.real).max() < 1e-11
+    assert np.abs(b - P1.real).max() < 1e-11
