@@ -1,0 +1,3 @@
+/* BasicOperators.h -- the reference's include/BasicOperators.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

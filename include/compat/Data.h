@@ -1,0 +1,3 @@
+/* Data.h -- the reference's include/Data.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

@@ -1,0 +1,3 @@
+/* Head.h -- the reference's include/Head.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

@@ -1,0 +1,3 @@
+/* Initialization.h -- the reference's include/Initialization.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

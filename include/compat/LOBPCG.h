@@ -1,0 +1,3 @@
+/* LOBPCG.h -- the reference's include/LOBPCG.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

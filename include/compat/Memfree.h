@@ -1,0 +1,3 @@
+/* Memfree.h -- the reference's include/Memfree.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

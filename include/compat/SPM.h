@@ -1,0 +1,3 @@
+/* SPM.h -- the reference's include/SPM.h name, served by the MI355X compat layer. */
+#pragma once
+#include "spm_reference_api.h"

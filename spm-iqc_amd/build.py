@@ -57,9 +57,30 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
     os.replace(tmp, LIB)
+    _build_compat()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+COMPAT_LIB = os.path.join(LIBDIR, "libspm_iqc_compat.so")
+STAGE1 = os.path.join(LIBDIR, "spm_stage1")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+
+
+def _build_compat():
+    """The reference-signature C++ layer over the C ABI, and the stage-1 driver."""
+    src = os.path.join(HERE, "compat")
+    host = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include")]
+    _run(host + ["-shared", "-o", COMPAT_LIB, os.path.join(src, "compat.cpp"), "-L", LIBDIR, "-lspm_iqc",
+                 "-Wl,-rpath,$ORIGIN"])
+    _run(host + ["-o", STAGE1, os.path.join(src, "main_spm.cpp"), "-L", LIBDIR, "-lspm_iqc_compat", "-lspm_iqc",
+                 "-Wl,-rpath,$ORIGIN"])
 
 
 if __name__ == "__main__":

@@ -1,0 +1,648 @@
+// compat.cpp -- the reference's C++ interface (include/compat/spm_reference_api.h)
+// implemented over the C ABI of libspm_iqc.so.
+//
+// The globals of Data.h point at DEVICE buffers of N^3 interleaved complex
+// doubles (the reference's fftw_complex layout).  Each solver call takes the
+// real part of its fields into the library's real layout, runs there, and
+// writes the result back with a zero imaginary part.  The device context is
+// created by My_mpi_fft_set_up and re-created whenever N, L, Tau, Gamma or dt
+// change, because the reference reads those globals at every call.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/compat/spm_reference_api.h"
+#include "../../include/spm_iqc.h"
+
+// ----------------------------------------------------------------- globals (Data.cpp:1-29)
+int *NCpt, *K, dimCpt, dimPhy, max_iter;
+double L;
+int **kindex;
+double **ProjMatrix;
+double Gamma, Tau, q, dt;
+fftw_complex *u_F, *u_R;
+fftw_complex *data;
+fftw_complex *u_R_local, *u_F_local, *u2_R_local, *u3_R_local, *u4_R_local;
+fftw_complex *nln_R_local, *nln_F_local, *Laplas_R_local, *Laplas_F_local;
+fftw_complex *fftw_Ctmp_local, *fftw_Rtmp_local;
+int **kindex_local;
+double **PK_local, *k_square_local, *epmckt2_local, *ikt2_local;
+int myid, numprocs;
+int N, CplxDofs;
+double PI;
+int CplxDofs_local;
+ptrdiff_t alloc_local, local_n0, local_0_start;
+int Ini_Phase;
+fftw_complex **Phi0, **dPhi;
+fftw_complex *tv;
+fftw_plan plan1, plan2;
+Option *option1, *option2;
+
+// ----------------------------------------------------------------- internals
+namespace {
+
+spm_ctx *g_ctx = nullptr;
+double g_key[5];                   // N, L, Tau, Gamma, dt of g_ctx
+double *g_re[4] = {nullptr, nullptr, nullptr, nullptr}; // device real scratch fields
+double *g_ones = nullptr;
+double *g_blk = nullptr;           // LOBPCG block (k real fields)
+int g_blk_k = 0;
+int *g_kflat = nullptr, *g_kflat_local = nullptr;
+double *g_pkflat = nullptr;
+
+[[noreturn]] void die(const char *what, int rc)
+{
+    std::fprintf(stderr, "spm compat: %s failed (%d): %s\n", what, rc, spm_last_error());
+    std::abort();
+}
+#define CK(x)                                                                                       \
+    do {                                                                                            \
+        int _r = (x);                                                                               \
+        if (_r) die(#x, _r);                                                                        \
+    } while (0)
+void hck(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "spm compat: %s: %s\n", what, hipGetErrorString(e));
+        std::abort();
+    }
+}
+
+size_t R() { return (size_t)CplxDofs_local; }
+
+void free_scratch()
+{
+    for (double *&p : g_re)
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+    if (g_ones) (void)hipFree(g_ones);
+    if (g_blk) (void)hipFree(g_blk);
+    g_ones = g_blk = nullptr;
+    g_blk_k = 0;
+}
+
+// The device context for the current globals.
+spm_ctx *ctx()
+{
+    const double key[5] = {(double)N, L, Tau, Gamma, dt};
+    if (g_ctx && std::memcmp(key, g_key, sizeof key) == 0) return g_ctx;
+    if (g_ctx) {
+        spm_destroy(g_ctx);
+        g_ctx = nullptr;
+        free_scratch();
+    }
+    spm_params p;
+    p.N = N;
+    p.precision = SPM_FP64;
+    p.L = L;
+    // ProjMatrix[j][j] = kscale / L (Initialization.cpp:35-38 has kscale = 1)
+    p.kscale = ProjMatrix ? ProjMatrix[0][0] * L : 1.0;
+    p.Tau = Tau;
+    p.Gamma = Gamma;
+    p.dt = dt;
+    CK(spm_create(&p, &g_ctx));
+    CK(spm_set_stream(g_ctx, nullptr)); // legacy stream: ordered with hipMemcpy
+    std::memcpy(g_key, key, sizeof key);
+    const size_t b = sizeof(double) * (size_t)N * N * N;
+    for (double *&p2 : g_re) hck(hipMalloc(&p2, b), "hipMalloc scratch");
+    std::vector<double> ones((size_t)N * N * N, 1.0);
+    hck(hipMalloc(&g_ones, b), "hipMalloc ones");
+    hck(hipMemcpy(g_ones, ones.data(), b, hipMemcpyHostToDevice), "upload ones");
+    return g_ctx;
+}
+
+double *block(int k)
+{
+    if (g_blk_k < k) {
+        if (g_blk) (void)hipFree(g_blk);
+        hck(hipMalloc(&g_blk, sizeof(double) * k * R()), "hipMalloc block");
+        g_blk_k = k;
+    }
+    return g_blk;
+}
+
+void to_real(const fftw_complex *z, double *x) { CK(spm_cplx_to_real(ctx(), z, x)); }
+void to_cplx(const double *x, fftw_complex *z) { CK(spm_real_to_cplx(ctx(), x, z)); }
+
+fftw_complex *dev_field()
+{
+    fftw_complex *p;
+    hck(hipMalloc(&p, sizeof(fftw_complex) * R()), "hipMalloc field");
+    hck(hipMemset(p, 0, sizeof(fftw_complex) * R()), "hipMemset field");
+    return p;
+}
+
+// a1 F1 + a2 F2 on n complex values = 2n doubles, done as two halves of the
+// context's field length (n must equal CplxDofs_local)
+void lin2_cplx(fftw_complex *r, double a1, const fftw_complex *F1, double a2, const fftw_complex *F2, int n)
+{
+    if ((size_t)n != R()) {
+        std::fprintf(stderr, "spm compat: vector length %d != CplxDofs_local %d\n", n, CplxDofs_local);
+        std::abort();
+    }
+    for (int h = 0; h < 2; ++h)
+        CK(spm_lin2(ctx(), (double *)r + h * R(), a1, (const double *)F1 + h * R(), a2,
+                    (const double *)F2 + h * R()));
+}
+
+FILE *open_or_die(const char *name, const char *mode)
+{
+    FILE *fp = std::fopen(name, mode);
+    if (!fp) {
+        std::fprintf(stderr, "spm compat: cannot open %s\n", name);
+        std::abort();
+    }
+    return fp;
+}
+
+} // namespace
+
+// ----------------------------------------------------------------- Initialization
+// Initialization.cpp:7-42.  N may be overridden with the environment variable
+// SPM_N (the reference hard-codes 256); L, Tau, Gamma, dt and ProjMatrix can be
+// changed through the globals before My_mpi_fft_set_up.
+void LPsystParameter()
+{
+    Ini_Phase = 0;
+    N = 256;
+    if (const char *e = std::getenv("SPM_N")) N = std::atoi(e);
+    CplxDofs = N * N * N;
+    PI = 3.14159265358979311599796346854418516;
+    dimCpt = 3, dimPhy = 3, max_iter = 100000, L = 26;
+    dt = 0.3;
+    Tau = -0.0061;
+    Gamma = 0.30;
+    if (myid == 0) {
+        std::printf("\nset parameter: Tau = %f, Gamma = %f \n", Tau, Gamma);
+        std::printf("\nset parameter: L = %f, N = %d \n", L, N);
+    }
+    q = 0;
+    ProjMatrix = (double **)std::malloc(sizeof(double *) * dimPhy);
+    for (int i = 0; i < dimPhy; i++) ProjMatrix[i] = (double *)std::calloc(dimCpt, sizeof(double));
+    for (int j = 0; j < dimCpt; j++) ProjMatrix[j][j] = 1.0 / L;
+    NCpt = (int *)std::malloc(sizeof(int) * dimCpt);
+    for (int i = 0; i < dimCpt; i++) NCpt[i] = N;
+}
+
+// Initialization.cpp:80-91 (one block instead of N^3 small allocations)
+void memAllocation_all()
+{
+    K = (int *)std::malloc(sizeof(int) * N);
+    g_kflat = (int *)std::malloc(sizeof(int) * 3 * (size_t)CplxDofs);
+    kindex = (int **)std::malloc(sizeof(int *) * (size_t)CplxDofs);
+    for (size_t i = 0; i < (size_t)CplxDofs; i++) kindex[i] = g_kflat + 3 * i;
+    u_F = (fftw_complex *)std::calloc((size_t)CplxDofs, sizeof(fftw_complex));
+    u_R = (fftw_complex *)std::calloc((size_t)CplxDofs, sizeof(fftw_complex));
+}
+
+// Initialization.cpp:95-121
+void get_kindex_all()
+{
+    for (int i = 0; i < N; i++) K[i] = i < N / 2 ? i : i - N;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            for (int k = 0; k < N; k++) {
+                const size_t l = ((size_t)i * N + j) * N + k;
+                kindex[l][0] = K[i];
+                kindex[l][1] = K[j];
+                kindex[l][2] = K[k];
+            }
+}
+
+// Initialization.cpp:44-78: field buffers on the device, k-space tables on the host
+void memAllocation_local()
+{
+    const size_t n = R();
+    g_kflat_local = (int *)std::malloc(sizeof(int) * 3 * n);
+    g_pkflat = (double *)std::malloc(sizeof(double) * 3 * n);
+    kindex_local = (int **)std::malloc(sizeof(int *) * n);
+    PK_local = (double **)std::malloc(sizeof(double *) * n);
+    for (size_t i = 0; i < n; i++) {
+        kindex_local[i] = g_kflat_local + 3 * i;
+        PK_local[i] = g_pkflat + 3 * i;
+    }
+    k_square_local = (double *)std::calloc(n, sizeof(double));
+    epmckt2_local = (double *)std::calloc(n, sizeof(double));
+    ikt2_local = (double *)std::calloc(n, sizeof(double));
+    u_F_local = dev_field();
+    u_R_local = dev_field();
+    u2_R_local = dev_field();
+    u3_R_local = dev_field();
+    u4_R_local = dev_field();
+    nln_R_local = dev_field();
+    nln_F_local = dev_field();
+    Laplas_R_local = dev_field();
+    Laplas_F_local = dev_field();
+    fftw_Ctmp_local = dev_field();
+    fftw_Rtmp_local = dev_field();
+    Phi0 = (fftw_complex **)std::malloc(sizeof(fftw_complex *) * 2);
+    for (int i = 0; i < 2; i++) Phi0[i] = dev_field();
+    tv = dev_field();
+}
+
+// Initialization.cpp:123-129
+void get_ikt2_local()
+{
+    for (int i = 0; i < CplxDofs_local; i++) ikt2_local[i] = 1.0 / (epmckt2_local[i] + std::fabs(Tau) + 0.02);
+}
+
+// Initialization.cpp:132-148 (k_square_local starts from zero)
+void get_ksquare_local()
+{
+    for (int l = 0; l < CplxDofs_local; l++) {
+        k_square_local[l] = 0.0;
+        for (int i = 0; i < dimPhy; i++) {
+            double tmp = 0.0;
+            for (int j = 0; j < dimCpt; j++) tmp = tmp + ProjMatrix[i][j] * kindex_local[l][j];
+            PK_local[l][i] = tmp;
+            k_square_local[l] = k_square_local[l] + std::pow(PK_local[l][i], 2);
+        }
+    }
+}
+
+double get_GK(double k2) { return spm_get_GK(k2); }
+
+// Initialization.cpp:168-174
+void get_epmckt2_local()
+{
+    for (int i = 0; i < CplxDofs_local; i++) epmckt2_local[i] = get_GK(k_square_local[i]);
+}
+
+// Initialization.cpp:176-205 (one rank: the local slab is the whole grid)
+void Distribute_C(fftw_complex *A, fftw_complex *a)
+{
+    hck(hipMemcpy(a, A + (size_t)local_0_start * N * N, sizeof(fftw_complex) * R(), hipMemcpyHostToDevice),
+        "Distribute_C");
+}
+
+void Distribute_C(int **A, int **a)
+{
+    for (size_t l = 0; l < R(); ++l)
+        for (int d = 0; d < 3; ++d) a[l][d] = A[(size_t)local_0_start * N * N + l][d];
+}
+
+void Collect_C(fftw_complex *a, fftw_complex *A)
+{
+    hck(hipMemcpy(A, a, sizeof(fftw_complex) * R(), hipMemcpyDeviceToHost), "Collect_C");
+}
+
+// Initialization.cpp:242-338 (host; u_F is the whole-grid host field)
+void get_initial_value(fftw_complex *v, int choice, double angle)
+{
+    if (myid == 0) std::printf("\nchoice = %d \t angle = %.3e ", choice, angle);
+    static const char *names[] = {"DIS", "BCC", "FCC", "HPC", "LAM", "DG"};
+    static const double lens[] = {1.0, std::sqrt(2.0), std::sqrt(3.0), std::sqrt(2.0), 1.0, std::sqrt(6.0)};
+    if (choice < 0 || choice > 5) {
+        std::fprintf(stderr, "spm compat: get_initial_value: choice %d not in 0..5\n", choice);
+        std::abort();
+    }
+    if (myid == 0) std::printf("\n--------------------%s----------------------------\n\n", names[choice]);
+    const double l = lens[choice];
+    for (int i = 0; i < CplxDofs; i++) v[i][0] = v[i][1] = 0.0;
+    const std::string path = std::string("./initData/") + names[choice];
+    FILE *fp = open_or_die(path.c_str(), "r");
+    int n = 0;
+    if (std::fscanf(fp, "%d", &n) != 1) n = 0;
+    std::vector<int> f(4 * (size_t)n, 0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 4; j++)
+            if (std::fscanf(fp, "%d", &f[4 * i + j]) != 1) f[4 * i + j] = 0;
+    std::fclose(fp);
+    for (int i = 0; i < n; i++) {
+        int k1 = (int)std::round(1.0 * f[4 * i + 0] / l * L);
+        int k2 = (int)std::round(1.0 * f[4 * i + 1] / l * L);
+        int k3 = (int)std::round(1.0 * f[4 * i + 2] / l * L);
+        if (k1 < 0) k1 = N + k1;
+        if (k2 < 0) k2 = N + k2;
+        if (k3 < 0) k3 = N + k3;
+        v[((size_t)k1 * N + k2) * N + k3][0] = f[4 * i + 3] * 0.1;
+    }
+}
+
+// ----------------------------------------------------------------- My_FFT (My_FFT.cpp)
+void My_mpi_fft_set_up()
+{
+    alloc_local = CplxDofs;
+    local_n0 = N;
+    local_0_start = 0;
+    CplxDofs_local = (int)(local_n0 * N * N);
+    numprocs = numprocs > 0 ? numprocs : 1;
+    ctx();
+    data = dev_field();
+    plan1 = plan2 = nullptr;
+}
+
+void my_fft(fftw_complex *x_R_local, fftw_complex *x_F_local) { CK(spm_my_fft(ctx(), x_R_local, x_F_local)); }
+void my_ifft(fftw_complex *x_F_local, fftw_complex *x_R_local) { CK(spm_my_ifft(ctx(), x_F_local, x_R_local)); }
+
+// ----------------------------------------------------------------- LPsrc
+// get_Ene (LPsrc.cpp:8-38) of the field in u_R_local (u_F_local = FFT of it).
+double get_Ene()
+{
+    double E = 0.0;
+    to_real(u_R_local, g_re[0]);
+    CK(spm_energy(ctx(), g_re[0], &E));
+    return E;
+}
+
+// get_min_u_F (LPsrc.cpp:40-117) on (u_R_local, u_F_local)
+void get_min_u_F()
+{
+    to_real(u_R_local, g_re[0]);
+    std::vector<double> hist(3 * (size_t)(max_iter > 0 ? max_iter : 1));
+    int it = 0;
+    CK(spm_gradflow(ctx(), g_re[0], max_iter, 1e-16, 0.0, 1000, hist.data(), &it));
+    if (myid == 0)
+        for (int i = 0; i < it; i += 10)
+            std::printf("\nmyid : %d ,\t iter : %d \t res: %.3e \t E: %.3e \t", myid, i, hist[3 * i + 2], hist[3 * i]);
+    to_cplx(g_re[0], u_R_local);
+    my_fft(u_R_local, u_F_local);
+}
+
+void ngrad_cam(fftw_complex *phir, fftw_complex *gradr)
+{
+    to_real(phir, g_re[0]);
+    CK(spm_ngrad(ctx(), g_re[0], g_re[1], nullptr));
+    to_cplx(g_re[1], gradr);
+}
+
+// ----------------------------------------------------------------- LOBPCG
+void hv_cam(fftw_complex *phir, fftw_complex *v, fftw_complex *hv)
+{
+    to_real(phir, g_re[0]);
+    to_real(v, g_re[1]);
+    CK(spm_hv(ctx(), g_re[0], g_re[1], g_re[2]));
+    to_cplx(g_re[2], hv);
+}
+
+void precond_camnew(fftw_complex *w)
+{
+    to_real(w, g_re[0]);
+    CK(spm_precond(ctx(), g_re[0]));
+    to_cplx(g_re[0], w);
+}
+
+// LOBPCG.cpp:10-109; the random block comes from a fixed-seed generator
+// (the reference uses Eigen's MatrixXd::Random)
+void hiosd_lobpcg_initialization(Option *option, int ki)
+{
+    option->k = ki;
+    option->maxiter = 21;
+    option->dt = 0.1;
+    option->epsf = 1e-7;
+    option->betat = 1.0;
+    option->betau = 1.0;
+    option->outputp = 1;
+    option->outputd = 1;
+    option->V = (fftw_complex **)std::malloc(sizeof(fftw_complex *) * ki);
+    for (int i = 0; i < ki; i++) option->V[i] = dev_field();
+    std::mt19937_64 gen(20240607u);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    std::vector<double> st((size_t)ki * R());
+    for (double &x : st) x = U(gen);
+    double *blk = block(ki);
+    hck(hipMemcpy(blk, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice), "upload start");
+    CK(spm_lobpcg_init(ctx(), blk, blk, ki));
+    for (int i = 0; i < ki; i++) to_cplx(blk + i * R(), option->V[i]);
+}
+
+// LOBPCG.cpp:207-487
+void hiosd_lobpcg_inip(fftw_complex *x, Option *option)
+{
+    const int k = option->k;
+    to_real(x, g_re[0]);
+    double *blk = block(k);
+    for (int i = 0; i < k; i++) to_real(option->V[i], blk + i * R());
+    std::vector<double> ritz(k);
+    CK(spm_lobpcg(ctx(), g_re[0], blk, k, option->maxiter, ritz.data()));
+    for (int i = 0; i < k; i++) to_cplx(blk + i * R(), option->V[i]);
+    if (myid == 0) {
+        std::printf("\n%d ", option->maxiter - 1);
+        for (int i = 0; i < k; i++) std::printf(" %.10g", ritz[i]);
+        std::printf("\n\n");
+    }
+}
+
+void copy_option_V(Option *o1, Option *o2, int k)
+{
+    for (int i = 0; i < k; i++) {
+        to_real(o1->V[i], g_re[0]);
+        to_cplx(g_re[0], o2->V[i]);
+    }
+}
+void copy_option_V(fftw_complex **V, Option *o2, int k)
+{
+    for (int i = 0; i < k; i++) {
+        to_real(V[i], g_re[0]);
+        to_cplx(g_re[0], o2->V[i]);
+    }
+}
+void copy_option_V(fftw_complex *v, Option *o2, int k)
+{
+    to_real(v, g_re[0]);
+    to_cplx(g_re[0], o2->V[k]);
+}
+
+// ----------------------------------------------------------------- SPM
+void get_Tv(fftw_complex **Phi, fftw_complex *v)
+{
+    to_real(Phi[0], g_re[0]);
+    to_real(Phi[1], g_re[1]);
+    CK(spm_get_tv(ctx(), g_re[0], g_re[1], g_re[2]));
+    to_cplx(g_re[2], v);
+}
+
+void Natural_Forces(fftw_complex *x, fftw_complex *v)
+{
+    to_real(x, g_re[0]);
+    to_real(v, g_re[1]);
+    CK(spm_natural_force(ctx(), g_re[0], g_re[1]));
+    to_cplx(g_re[0], x);
+}
+
+void MinMax_Forces(fftw_complex *x, fftw_complex *v)
+{
+    to_real(x, g_re[0]);
+    to_real(v, g_re[1]);
+    CK(spm_minmax_force(ctx(), g_re[0], g_re[1]));
+    to_cplx(g_re[0], x);
+}
+
+// SPM.cpp:170-263; leaves the final tangent in the global tv as the reference does
+void Spring_Pairing(fftw_complex **Phi, double maxiter, double tol)
+{
+    to_real(Phi[0], g_re[0]);
+    to_real(Phi[1], g_re[1]);
+    const int mi = (int)maxiter;
+    std::vector<double> hist(mi > 0 ? mi : 1);
+    int it = 0;
+    double res = 0.0;
+    CK(spm_spring_pairing(ctx(), g_re[0], g_re[1], mi, tol, &it, &res, hist.data()));
+    if (myid == 0)
+        for (int i = 10; i <= it; i += 10) std::printf("\n iter = %d , res =  %.3e \n", i, hist[i - 1]);
+    to_cplx(g_re[0], Phi[0]);
+    to_cplx(g_re[1], Phi[1]);
+    get_Tv(Phi, tv);
+}
+
+// ----------------------------------------------------------------- BasicOperators
+void FuncsLinear1Cplx(fftw_complex *rslt, int n, const double a1, const fftw_complex *F1)
+{
+    lin2_cplx(rslt, a1, F1, 0.0, F1, n);
+}
+void FuncsLinear2Cplx(fftw_complex *rslt, int n, const double a1, const fftw_complex *F1, const double a2,
+                      const fftw_complex *F2)
+{
+    lin2_cplx(rslt, a1, F1, a2, F2, n);
+}
+void Cpy(fftw_complex *a, fftw_complex *b, int n)
+{
+    hck(hipMemcpy(b, a, sizeof(fftw_complex) * (size_t)n, hipMemcpyDeviceToDevice), "Cpy");
+}
+void FFT_dot_constant(fftw_complex *rslt, int n, double a1) { lin2_cplx(rslt, a1, rslt, 0.0, rslt, n); }
+void setCplxZero_v(fftw_complex *x, int cplxDofs)
+{
+    hck(hipMemset(x, 0, sizeof(fftw_complex) * (size_t)cplxDofs), "setCplxZero_v");
+}
+double mean_Cplx(fftw_complex *src, int n)
+{
+    if ((size_t)n != R()) die("mean_Cplx: n != CplxDofs_local", SPM_EINVAL);
+    double m = 0.0;
+    to_real(src, g_re[0]);
+    CK(spm_inpfp(ctx(), g_re[0], g_ones, &m));
+    return m;
+}
+double inpfp(fftw_complex *p, fftw_complex *qv)
+{
+    double s = 0.0;
+    to_real(p, g_re[0]);
+    to_real(qv, g_re[1]);
+    CK(spm_inpfp(ctx(), g_re[0], g_re[1], &s));
+    return s;
+}
+double nrmp(fftw_complex *p) { return std::sqrt(inpfp(p, p)); }
+double rayleighq(fftw_complex *x, fftw_complex *v)
+{
+    double s = 0.0;
+    to_real(x, g_re[0]);
+    to_real(v, g_re[1]);
+    CK(spm_rayleighq(ctx(), g_re[0], g_re[1], &s));
+    return s;
+}
+
+// ----------------------------------------------------------------- Memfree (Memfree.cpp:5-73)
+void Memfree_global_kspace()
+{
+    std::free(K);
+    std::free(kindex);
+    std::free(g_kflat);
+    K = nullptr;
+    kindex = nullptr;
+    g_kflat = nullptr;
+}
+
+void Memfree_global_data()
+{
+    std::free(u_R);
+    std::free(u_F);
+    std::free(NCpt);
+    for (int i = 0; i < dimPhy; i++) std::free(ProjMatrix[i]);
+    std::free(ProjMatrix);
+    u_R = u_F = nullptr;
+    ProjMatrix = nullptr;
+}
+
+void Memfree_local_kspace()
+{
+    std::free(PK_local);
+    std::free(kindex_local);
+    std::free(g_kflat_local);
+    std::free(g_pkflat);
+    std::free(k_square_local);
+    std::free(epmckt2_local);
+    std::free(ikt2_local);
+}
+
+void Memfree_local_data()
+{
+    fftw_complex *fs[] = {u_R_local, u_F_local, u2_R_local, u3_R_local, u4_R_local, nln_R_local, nln_F_local,
+                          Laplas_R_local, Laplas_F_local, fftw_Ctmp_local, fftw_Rtmp_local, tv, data};
+    for (fftw_complex *p : fs)
+        if (p) (void)hipFree(p);
+    for (int i = 0; i < 2; i++) (void)hipFree(Phi0[i]);
+    std::free(Phi0);
+    free_scratch();
+    if (g_ctx) spm_destroy(g_ctx);
+    g_ctx = nullptr;
+}
+
+void Memfree_Option(Option *option)
+{
+    for (int i = 0; i < option->k; i++) (void)hipFree(option->V[i]);
+    std::free(option->V);
+    std::free(option);
+}
+
+// ----------------------------------------------------------------- File_Op (File_Op.cpp:8-89)
+void Save_Option_V(Option *option, char *filename)
+{
+    std::printf("\n------------Save---V----------------------------\n");
+    std::vector<fftw_complex> v(R());
+    FILE *fp = open_or_die(filename, "w");
+    for (int i = 0; i < option->k; i++) {
+        hck(hipMemcpy(v.data(), option->V[i], sizeof(fftw_complex) * R(), hipMemcpyDeviceToHost), "Save_Option_V");
+        for (size_t j = 0; j < R(); j++) std::fprintf(fp, "%.16e \n", v[j][0]);
+    }
+    std::fclose(fp);
+}
+
+void Save_phi(fftw_complex *phi, char *filename)
+{
+    std::printf("\n--------------save-----------------\n");
+    FILE *fp = open_or_die(filename, "w");
+    for (int i = 0; i < CplxDofs; i++) std::fprintf(fp, "%.16e \n", phi[i][0]);
+    std::fclose(fp);
+}
+
+void load_phi(fftw_complex *x, char *filename)
+{
+    std::printf("\n -----------loading------------------");
+    FILE *fp = open_or_die(filename, "r");
+    for (int i = 0; i < CplxDofs; i++) {
+        if (std::fscanf(fp, "%lf", &(x[i][0])) != 1) die("load_phi: short file", SPM_EINVAL);
+        x[i][1] = 0.0;
+    }
+    std::fclose(fp);
+}
+
+void load_Option_V(Option *option, char *filename)
+{
+    std::printf("\n-----------------load-V----------------------");
+    std::vector<fftw_complex> v(R());
+    FILE *fp = open_or_die(filename, "r");
+    for (int i = 0; i < option->k; i++) {
+        for (size_t j = 0; j < R(); j++) {
+            if (std::fscanf(fp, "%lf", &(v[j][0])) != 1) die("load_Option_V: short file", SPM_EINVAL);
+            v[j][1] = 0.0;
+        }
+        hck(hipMemcpy(option->V[i], v.data(), sizeof(fftw_complex) * R(), hipMemcpyHostToDevice), "load_Option_V");
+    }
+    std::fclose(fp);
+}
+
+double spm_compat_wtime()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+spm_ctx *spm_compat_context() { return g_ctx; }
