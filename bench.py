@@ -37,6 +37,31 @@ def algorithmic_bytes_per_step(N: int, s: int) -> int:
     return 2 * R * s + 5 * C * 2 * s
 
 
+STEP_KERNELS = ("SOpGradFlowStep", "ProGradFlowReduce", "ProGradFlowFinalize", "true, false, true, SOpNone")
+
+
+def measured_traffic(N, dtype):
+    """HBM bytes of one step, summed over its kernels, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/gradflow_step_<N>.md);
+    None if there is none for this size."""
+    path = os.path.join(ROOT, "profiles", "gradflow_step_%d.md" % N)
+    if not os.path.exists(path):
+        return None
+    total, seen = 0.0, set()
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) < 7 or not cells[1].startswith("`k_"):
+            continue
+        name = cells[1]
+        if ("<%s, %d," % (dtype, N)) not in name:
+            continue
+        for k in STEP_KERNELS:
+            if k in name and k not in seen and ("ProNone" in name or k != "true, false, true, SOpNone"):
+                seen.add(k)
+                total += (float(cells[4]) + float(cells[5])) * 1e6
+    return total if len(seen) == len(STEP_KERNELS) else None
+
+
 def seed_field(seeds, N):
     # icosahedral 30-vector star, |q| = 1, in a sphere of radius 6 (reference metric k = n/L)
     return seeds.icosahedral_nucleus(N, L=26.0, kscale=1.0, amp=0.1, seed=0)
@@ -152,7 +177,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "gradient-flow step (5 launches: i-pass fwd+update+inv, "
                                                "finalize, j-pass inv, k-pass c2r+f+r2c, j-pass fwd)",
                      "achieved": round(ach64 / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(ach64 / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(ach64 / HBM_PEAK, 4), "traffic": measured_traffic(N, "double"),
                      "algorithmic_bytes_per_step": B64},
         "cpu_baseline": cpu,
         "wall_s": {"fp64": round(r64["wall_s"], 4), "fp32": round(r32["wall_s"], 4)},

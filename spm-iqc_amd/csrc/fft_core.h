@@ -144,32 +144,57 @@ template <int R, int DIR, typename T> __device__ __forceinline__ void dft(C2<T> 
 }
 
 // ---------------------------------------------------------------- plans
-// Supported lengths: TPP threads per pencil and the radix of each stage.
-// Every radix divides E = N / TPP, so each thread does E / R butterflies.
-template <int N> struct FftPlan;
-#define SPM_PLAN(N_, TPP_, NST_, R0_, R1_, R2_)                      \
-    template <> struct FftPlan<N_> {                                 \
-        static constexpr int TPP = TPP_, E = N_ / TPP_, NST = NST_;  \
-        static constexpr int R0 = R0_, R1 = R1_, R2 = R2_;           \
+// A length-N transform by TPP threads per pencil, E = N / TPP values each.
+// Stage radices are chosen greedily from {16, 12, 8, 6, 4, 3, 2}: each must
+// divide E (so a thread does E / R butterflies) and the length left.
+__host__ __device__ constexpr int pick_radix(int E, int rem)
+{
+    const int cand[7] = {16, 12, 8, 6, 4, 3, 2};
+    for (int i = 0; i < 7; ++i)
+        if (E % cand[i] == 0 && rem % cand[i] == 0) return cand[i];
+    return 0;
+}
+
+template <int N, int TPP_> struct FftPlan {
+    static constexpr int TPP = TPP_, E = N / TPP_;
+    static constexpr int R0 = pick_radix(E, N);
+    static constexpr int R1 = N / R0 > 1 ? pick_radix(E, N / R0) : 1;
+    static constexpr int R2 = N / (R0 * R1) > 1 ? pick_radix(E, N / (R0 * R1)) : 1;
+    static constexpr int R3 = N / (R0 * R1 * R2) > 1 ? pick_radix(E, N / (R0 * R1 * R2)) : 1;
+    static constexpr int NST = (R0 > 1) + (R1 > 1) + (R2 > 1) + (R3 > 1);
+    static_assert(N % TPP_ == 0 && R0 * R1 * R2 * R3 == N, "unsupported (N, threads per pencil)");
+};
+
+// Default threads per pencil for each supported N.
+template <int N> struct DefTPP;
+#define SPM_DEFTPP(N_, T_)                                                                                 \
+    template <> struct DefTPP<N_> {                                                                        \
+        static constexpr int value = T_;                                                                   \
     };
-SPM_PLAN(4, 1, 1, 4, 1, 1)
-SPM_PLAN(8, 2, 2, 4, 2, 1)
-SPM_PLAN(12, 2, 2, 6, 2, 1)
-SPM_PLAN(16, 4, 2, 4, 4, 1)
-SPM_PLAN(24, 2, 2, 12, 2, 1)
-SPM_PLAN(32, 4, 2, 8, 4, 1)
-SPM_PLAN(48, 4, 2, 12, 4, 1)
-SPM_PLAN(64, 4, 2, 16, 4, 1)
-SPM_PLAN(96, 8, 3, 12, 4, 2)
-SPM_PLAN(128, 8, 2, 16, 8, 1)
-SPM_PLAN(192, 16, 3, 12, 4, 4)
-SPM_PLAN(256, 16, 2, 16, 16, 1)
-#undef SPM_PLAN
+SPM_DEFTPP(4, 1)
+SPM_DEFTPP(8, 2)
+SPM_DEFTPP(12, 2)
+SPM_DEFTPP(16, 4)
+SPM_DEFTPP(24, 2)
+SPM_DEFTPP(32, 4)
+SPM_DEFTPP(48, 4)
+SPM_DEFTPP(64, 4)
+SPM_DEFTPP(96, 8)
+SPM_DEFTPP(128, 8)
+SPM_DEFTPP(192, 16)
+SPM_DEFTPP(256, 16)
+#undef SPM_DEFTPP
+
+// LDS index of pencil element e.  PAD inserts one slot after every 16
+// elements: the first stage writes element t*R + q from thread t, which
+// without padding puts every thread of a pencil in the same bank.
+template <bool PAD> __device__ __forceinline__ constexpr int lpad(int e) { return PAD ? e + (e >> 4) : e; }
+template <int N, bool PAD> constexpr int pencil_slots() { return PAD ? N + N / 16 : N; }
 
 // One Stockham stage.  v: the thread's E registers (cyclic layout on input).
-// lds: this pencil's exchange area, element e at lds[e * LS].
+// lds: this pencil's exchange area, element e at lds[lpad(e) * LS].
 // tw: exp(-2 pi i e / N), e = 0..N-1.
-template <typename T, int N, int TPP, int DIR, int R, int NS, bool LAST, int LS>
+template <typename T, int N, int TPP, int DIR, int R, int NS, bool LAST, int LS, bool PAD>
 __device__ __forceinline__ void stockham_stage(C2<T> (&v)[N / TPP], C2<T> *lds, const C2<T> *__restrict__ tw,
                                                int t)
 {
@@ -198,32 +223,36 @@ __device__ __forceinline__ void stockham_stage(C2<T> (&v)[N / TPP], C2<T> *lds, 
         } else {
             const int base = (j / NS) * NS * R + jm;
 #pragma unroll
-            for (int q = 0; q < R; ++q) lds[(base + q * NS) * LS] = a[q];
+            for (int q = 0; q < R; ++q) lds[lpad<PAD>(base + q * NS) * LS] = a[q];
         }
     }
     if constexpr (!LAST) {
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = lds[(t + TPP * m) * LS];
+        for (int m = 0; m < E; ++m) v[m] = lds[lpad<PAD>(t + TPP * m) * LS];
     }
 }
 
 // Full length-N transform of the pencil held in v (cyclic in, cyclic out).
 // Unnormalised in both directions.
-template <typename T, int N, int DIR, int LS>
-__device__ __forceinline__ void fft_pencil(C2<T> (&v)[N / FftPlan<N>::TPP], C2<T> *lds,
-                                           const C2<T> *__restrict__ tw, int t)
+template <typename T, int N, int TPP, int DIR, int LS, bool PAD>
+__device__ __forceinline__ void fft_pencil(C2<T> (&v)[N / TPP], C2<T> *lds, const C2<T> *__restrict__ tw, int t)
 {
-    using P = FftPlan<N>;
+    using P = FftPlan<N, TPP>;
     if constexpr (P::NST == 1) {
-        stockham_stage<T, N, P::TPP, DIR, P::R0, 1, true, LS>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R0, 1, true, LS, PAD>(v, lds, tw, t);
     } else if constexpr (P::NST == 2) {
-        stockham_stage<T, N, P::TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
-        stockham_stage<T, N, P::TPP, DIR, P::R1, P::R0, true, LS>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R0, 1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R1, P::R0, true, LS, PAD>(v, lds, tw, t);
+    } else if constexpr (P::NST == 3) {
+        stockham_stage<T, N, TPP, DIR, P::R0, 1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R1, P::R0, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R2, P::R0 * P::R1, true, LS, PAD>(v, lds, tw, t);
     } else {
-        stockham_stage<T, N, P::TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
-        stockham_stage<T, N, P::TPP, DIR, P::R1, P::R0, false, LS>(v, lds, tw, t);
-        stockham_stage<T, N, P::TPP, DIR, P::R2, P::R0 * P::R1, true, LS>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R0, 1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R1, P::R0, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R2, P::R0 * P::R1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage<T, N, TPP, DIR, P::R3, P::R0 * P::R1 * P::R2, true, LS, PAD>(v, lds, tw, t);
     }
 }
 

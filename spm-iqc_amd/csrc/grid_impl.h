@@ -7,16 +7,20 @@
 
 namespace spm {
 
-template <typename T, int N> struct GridImpl final : GridIface {
-    using G = Geo<T, N>;
+// TR, TJ, TI: threads per pencil of the row, j-axis and i-axis passes.
+template <typename T, int N, int TR = DefTPP<N>::value, int TJ = DefTPP<N>::value, int TI = DefTPP<N>::value>
+struct GridImpl final : GridIface {
+    using GR = Geo<T, N, TR>;
+    using GJ = Geo<T, N, TJ>;
+    using GI = Geo<T, N, TI>;
     using CT = C2<T>;
     static constexpr long NPAIRS = (long)N * N / 2; // row pairs of one field
 
     void geometry(long &nb_rows, long &nb_j, long &nb_i, int NHP) override
     {
-        nb_rows = (NPAIRS + G::PP - 1) / G::PP;
-        nb_j = (long)N * ((NHP + G::B - 1) / G::B);
-        nb_i = ((long)N * NHP + G::B - 1) / G::B;
+        nb_rows = (NPAIRS + GR::PP - 1) / GR::PP;
+        nb_j = (long)N * ((NHP + GJ::B - 1) / GJ::B);
+        nb_i = ((long)N * NHP + GI::B - 1) / GI::B;
     }
 
     // ---------------------------------------------------------- launchers
@@ -24,8 +28,8 @@ template <typename T, int N> struct GridImpl final : GridIface {
     static void rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl = nullptr,
                      Pro pro = Pro{})
     {
-        const long nb = (NPAIRS + G::PP - 1) / G::PP;
-        hipLaunchKernelGGL((k_rows<T, N, IN_SPEC, OUT_SPEC, Op, Pro>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0,
+        const long nb = (NPAIRS + GR::PP - 1) / GR::PP;
+        hipLaunchKernelGGL((k_rows<T, N, TR, IN_SPEC, OUT_SPEC, Op, Pro>), dim3((unsigned)nb), dim3(GR::ROW_THREADS), 0,
                            c.stream, in, (CT *)out, c.NHP, NPAIRS, (const CT *)c.tw, op, part, ctl, pro);
     }
     // pencils along j inside each i plane; row length nhp, valid nh
@@ -34,8 +38,8 @@ template <typename T, int N> struct GridImpl final : GridIface {
                        int nh = -1, int nhp = -1, Pro pro = Pro{})
     {
         if (nh < 0) { nh = c.NH; nhp = c.NHP; }
-        const long nb = (long)N * ((nhp + G::B - 1) / G::B);
-        hipLaunchKernelGGL((k_cols<T, N, FWD, INV, STORE, SOp, Pro>), dim3((unsigned)nb), dim3(G::COL_THREADS), 0,
+        const long nb = (long)N * ((nhp + GJ::B - 1) / GJ::B);
+        hipLaunchKernelGGL((k_cols<T, N, TJ, FWD, INV, STORE, SOp, Pro>), dim3((unsigned)nb), dim3(GJ::COL_THREADS), 0,
                            c.stream, (const CT *)src, (CT *)dst, (long)nhp, (long)nhp, (long)N * nhp, nh, nhp, 0,
                            (const CT *)c.tw, op, part, ctl, pro);
     }
@@ -46,8 +50,8 @@ template <typename T, int N> struct GridImpl final : GridIface {
     {
         if (nh < 0) { nh = c.NH; nhp = c.NHP; }
         const long ncols = (long)N * nhp;
-        const long nb = (ncols + G::B - 1) / G::B;
-        hipLaunchKernelGGL((k_cols<T, N, FWD, INV, STORE, SOp>), dim3((unsigned)nb), dim3(G::COL_THREADS), 0,
+        const long nb = (ncols + GI::B - 1) / GI::B;
+        hipLaunchKernelGGL((k_cols<T, N, TI, FWD, INV, STORE, SOp>), dim3((unsigned)nb), dim3(GI::COL_THREADS), 0,
                            c.stream, (const CT *)src, (CT *)dst, ncols, ncols, 0L, nh, nhp, 1, (const CT *)c.tw, op,
                            part, ctl);
     }
@@ -70,14 +74,14 @@ template <typename T, int N> struct GridImpl final : GridIface {
     void c2c(Ctx &c, const void *in, void *out, int dir) override
     {
         const long nrows = (long)N * N;
-        const long nb = (nrows + G::PP - 1) / G::PP;
+        const long nb = (nrows + GR::PP - 1) / GR::PP;
         if (dir < 0) {
-            hipLaunchKernelGGL((k_rows_c2c<T, N, -1>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0, c.stream,
+            hipLaunchKernelGGL((k_rows_c2c<T, N, TR, -1>), dim3((unsigned)nb), dim3(GR::ROW_THREADS), 0, c.stream,
                                (const CT *)in, (CT *)out, nrows, 1.0, (const CT *)c.tw);
             cols_j<true, false, true>(c, out, out, SOpNone{}, nullptr, nullptr, N, N);
             cols_i<true, false, true>(c, out, out, SOpScale<T>{1.0 / (double)c.R}, nullptr, nullptr, N, N);
         } else {
-            hipLaunchKernelGGL((k_rows_c2c<T, N, +1>), dim3((unsigned)nb), dim3(G::ROW_THREADS), 0, c.stream,
+            hipLaunchKernelGGL((k_rows_c2c<T, N, TR, +1>), dim3((unsigned)nb), dim3(GR::ROW_THREADS), 0, c.stream,
                                (const CT *)in, (CT *)out, nrows, 1.0, (const CT *)c.tw);
             cols_j<false, true, true>(c, out, out, SOpNone{}, nullptr, nullptr, N, N);
             cols_i<false, true, true>(c, out, out, SOpNone{}, nullptr, nullptr, N, N);

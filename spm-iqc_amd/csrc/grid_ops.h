@@ -35,8 +35,8 @@ struct Ctl {
     double gn_tol;  // stop when ||grad F|| <= gn_tol
 };
 
-template <typename T, int N> struct Geo {
-    static constexpr int TPP = FftPlan<N>::TPP, E = FftPlan<N>::E;
+template <typename T, int N, int TPP_ = DefTPP<N>::value> struct Geo {
+    static constexpr int TPP = TPP_, E = N / TPP_;
     // column passes: B adjacent columns per block -> 128-byte row segments
     static constexpr int B0 = 128 / (int)sizeof(C2<T>);
     static constexpr int B = (64 / TPP > B0) ? 64 / TPP : B0;
@@ -45,7 +45,7 @@ template <typename T, int N> struct Geo {
     static constexpr int PP0 = (sizeof(T) == 8 ? 128 : 256) / TPP;
     static constexpr int PP = PP0 > 0 ? PP0 : 1;
     static constexpr int ROW_THREADS = PP * TPP;
-    static constexpr int RSTR = N + 1; // LDS stride between rows (complex)
+    static constexpr int RSTR = pencil_slots<N, true>() + 1; // LDS stride between row pencils (complex)
 };
 
 __device__ __forceinline__ int kfold(int i, int N) { return i < N / 2 ? i : i - N; }
@@ -394,12 +394,12 @@ __device__ __forceinline__ void load_twiddles(C2<T> *stw, const C2<T> *__restric
 //   OUT_SPEC: forward r2c (unnormalised) of op(x) into out (row stride NHP)
 // The block's input rows are contiguous in memory: they are staged through
 // LDS with coalesced loads, each element read once.
-template <typename T, int N, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
-__global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
+template <typename T, int N, int TPP_, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
+__global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
     k_rows(const void *in, C2<T> *out, int NHP, long npairs, const C2<T> *__restrict__ tw, Op op,
            double *__restrict__ part, const Ctl *ctl, Pro pro = Pro{})
 {
-    using G = Geo<T, N>;
+    using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, PP = G::PP, NH = N / 2 + 1;
     constexpr int XCH = PP * G::RSTR, STG = IN_SPEC ? 2 * PP * NH : PP * N;
     __shared__ C2<T> lds[XCH > STG ? XCH : STG];
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
             // Z = Xa + i Xb on 0..N/2, conj(Xa) + i conj(Xb) above
             v[m] = p <= N / 2 ? C2<T>{a.x - b.y, a.y + b.x} : C2<T>{a.x + b.y, b.x - a.y};
         }
-        fft_pencil<T, N, +1, 1>(v, my, stw, t);
+        fft_pencil<T, N, TPP, +1, 1, true>(v, my, stw, t);
     } else {
         // 16-byte chunks of the block's 2*PP contiguous real rows
         constexpr int VPC = 16 / sizeof(T);
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
         }
     }
     if constexpr (OUT_SPEC) {
-        fft_pencil<T, N, -1, 1>(v, my, stw, t);
+        fft_pencil<T, N, TPP, -1, 1, true>(v, my, stw, t);
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m];
@@ -493,12 +493,12 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
 
 // Complex rows (the reference's full c2c layout): length-N transforms of
 // nrows contiguous rows, out = sc * DFT_DIR(in).
-template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
+template <typename T, int N, int TPP_, int DIR>
+__global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
     k_rows_c2c(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, long nrows, double sc,
                const C2<T> *__restrict__ tw)
 {
-    using G = Geo<T, N>;
+    using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, PP = G::PP;
     __shared__ C2<T> lds[PP * G::RSTR];
     __shared__ C2<T> stw[N];
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
     C2<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = active ? in[row * N + t + TPP * m] : C2<T>{0, 0};
-    fft_pencil<T, N, DIR, 1>(v, lds + pp * G::RSTR, stw, t);
+    fft_pencil<T, N, TPP, DIR, 1, true>(v, lds + pp * G::RSTR, stw, t);
     if (active)
 #pragma unroll
         for (int m = 0; m < E; ++m) out[row * N + t + TPP * m] = cscale(v[m], (T)sc);
@@ -520,13 +520,13 @@ __global__ void __launch_bounds__((Geo<T, N>::ROW_THREADS))
 // Valid columns: col < ncols and (col mod NHP) < NH.  iaxis: the pencil runs
 // along i (frequencies i = e, j = col / NHP, kz = col mod NHP); otherwise
 // along j (i = o, j = e, kz = col).
-template <typename T, int N, bool FWD, bool INV, bool STORE, class SOp, class Pro = ProNone>
-__global__ void __launch_bounds__((Geo<T, N>::COL_THREADS))
+template <typename T, int N, int TPP_, bool FWD, bool INV, bool STORE, class SOp, class Pro = ProNone>
+__global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
     k_cols(const C2<T> *src, C2<T> *dst, long pstride, long ncols, long ostride, int NH, int NHP,
            int iaxis, const C2<T> *__restrict__ tw, SOp op, double *__restrict__ part,
            const Ctl *ctl, Pro pro = Pro{})
 {
-    using G = Geo<T, N>;
+    using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, B = G::B;
     __shared__ C2<T> lds[N * B];
     __shared__ C2<T> stw[N];
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__((Geo<T, N>::COL_THREADS))
 #pragma unroll
         for (int m = 0; m < E; ++m) pre[m] = active ? ps[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
     }
-    if constexpr (FWD) fft_pencil<T, N, -1, B>(v, lds + c, stw, t);
+    if constexpr (FWD) fft_pencil<T, N, TPP, -1, B, false>(v, lds + c, stw, t);
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -564,7 +564,7 @@ __global__ void __launch_bounds__((Geo<T, N>::COL_THREADS))
             op(v[m], pre[SOp::PRELOAD ? m : 0], off + (long)e * pstride, i, j, kz, acc);
         }
     }
-    if constexpr (INV) fft_pencil<T, N, +1, B>(v, lds + c, stw, t);
+    if constexpr (INV) fft_pencil<T, N, TPP, +1, B, false>(v, lds + c, stw, t);
     if constexpr (STORE) {
         if (active)
 #pragma unroll
