@@ -50,6 +50,9 @@ typedef struct {
     double Tau;     /* -0.0061 */
     double Gamma;   /* 0.30 */
     double dt;      /* gradient-flow step (0.3) */
+    int shape;      /* FFT pass shape: 0 = built-in default, k = shape k of this N
+                       (spm_shapes), -1 = the tuned one (spm_tune; measured on
+                       first use and kept in the tuning cache) */
 } spm_params;
 
 typedef struct {
@@ -161,6 +164,55 @@ int spm_lin2(spm_ctx *c, void *r, double a1, const void *F1, double a2, const vo
 /* --- compat helpers: real part of a full complex field <-> real field */
 int spm_cplx_to_real(spm_ctx *c, const void *z, void *x);
 int spm_real_to_cplx(spm_ctx *c, const void *x, void *z);
+
+/* --- Batches of independent gradient flows (continuous batching)
+ *
+ * No counterpart in the reference, which relaxes one field per process
+ * (main_SPM.cpp:66-72, get_min_u_F LPsrc.cpp:40-117).  A batch holds
+ * `capacity` members of one grid size, precision, L, kscale and dt; each
+ * member is one get_min_u_F run with its own Tau, Gamma, limits and history
+ * (hist_cap iterations).  spm_batch_steps advances every running member: each
+ * pass of a step is one launch over all members, and the steps are replayed
+ * from a HIP graph.  Members can be loaded, read and released between calls
+ * while the others keep running.  Members compute exactly what spm_gradflow
+ * computes for the same field and parameters (the same kernels).
+ * A batch runs on a non-blocking stream of its own (spm_batch_stream). */
+typedef struct spm_batch spm_batch;
+int spm_batch_create(const spm_params *p, int capacity, int hist_cap, spm_batch **out);
+int spm_batch_destroy(spm_batch *b);
+int spm_batch_stream(const spm_batch *b, void **hip_stream);
+/* capacity, device bytes held (state, partial sums, histories) */
+int spm_batch_info(const spm_batch *b, int *capacity, size_t *dev_bytes);
+/* Member `slot` starts get_min_u_F from phi (device, read on the batch stream:
+ * finish writing it before the call and keep it until the next poll or read)
+ * with its own Tau, Gamma and limits as spm_gradflow (max_iter <= hist_cap). */
+int spm_batch_load(spm_batch *b, int slot, const void *phi, double Tau, double Gamma, int max_iter,
+                   double res_tol, double gn_tol);
+/* Enqueue nsteps iterations of every running member (asynchronous). */
+int spm_batch_steps(spm_batch *b, int nsteps);
+/* Synchronise; iters and stopped (host, capacity entries each, either may be
+ * NULL) receive each member's iteration count and stop flag.  Returns the
+ * number of running members (>= 0) or an error code. */
+int spm_batch_poll(spm_batch *b, int *iters, int *stopped);
+/* Synchronise; phi (device, may be NULL) <- the member's field; hist (host,
+ * may be NULL, 3 * *iters doubles) <- its history as spm_gradflow's. */
+int spm_batch_read(spm_batch *b, int slot, void *phi, double *hist, int *iters);
+/* Stop the member (its slot can be loaded again). */
+int spm_batch_release(spm_batch *b, int slot);
+
+/* --- FFT pass shapes and the tuner (the counterpart of My_mpi_fft_set_up's
+ * FFTW_PATIENT planning with wisdom files, My_FFT.cpp:45-97).
+ * A shape is the number of threads per pencil of the row (k axis), j-axis and
+ * i-axis passes; it fixes the radices of each pencil FFT and the block tiles. */
+/* Shapes available for N: count, and (if tpp != NULL) 3 ints per shape. */
+int spm_shapes(int N, int *count, int *tpp, int max_shapes);
+/* Time `steps` gradient-flow iterations of a batch of nmem members for every
+ * shape of p->N in p->precision; *best <- the fastest; us_per_step (host,
+ * may be NULL, one per shape) <- the times.  The result is recorded in the
+ * tuning cache (the file named by SPM_TUNE_CACHE, default
+ * $HOME/.cache/spm-iqc/shapes.txt), which spm_create and spm_batch_create
+ * read for shape -1. */
+int spm_tune(const spm_params *p, int nmem, int steps, int *best, double *us_per_step);
 
 #ifdef __cplusplus
 }

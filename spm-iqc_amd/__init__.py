@@ -37,7 +37,7 @@ class SpmError(RuntimeError):
 class _Params(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("precision", ctypes.c_int), ("L", ctypes.c_double),
                 ("kscale", ctypes.c_double), ("Tau", ctypes.c_double), ("Gamma", ctypes.c_double),
-                ("dt", ctypes.c_double)]
+                ("dt", ctypes.c_double), ("shape", ctypes.c_int)]
 
 
 class _Layout(ctypes.Structure):
@@ -85,6 +85,17 @@ SIGNATURES = {
     "spm_lin2": (_i, [_c, _c, _d, _c, _d, _c]),
     "spm_cplx_to_real": (_i, [_c, _c, _c]),
     "spm_real_to_cplx": (_i, [_c, _c, _c]),
+    "spm_batch_create": (_i, [ctypes.POINTER(_Params), _i, _i, ctypes.POINTER(_c)]),
+    "spm_batch_destroy": (_i, [_c]),
+    "spm_batch_stream": (_i, [_c, ctypes.POINTER(_c)]),
+    "spm_batch_info": (_i, [_c, _pi, ctypes.POINTER(ctypes.c_size_t)]),
+    "spm_batch_load": (_i, [_c, _i, _c, _d, _d, _i, _d, _d]),
+    "spm_batch_steps": (_i, [_c, _i]),
+    "spm_batch_poll": (_i, [_c, _c, _c]),
+    "spm_batch_read": (_i, [_c, _i, _c, _c, _pi]),
+    "spm_batch_release": (_i, [_c, _i]),
+    "spm_shapes": (_i, [_i, _pi, _c, _i]),
+    "spm_tune": (_i, [ctypes.POINTER(_Params), _i, _i, _pi, _c]),
 }
 
 
@@ -121,6 +132,31 @@ def _torch():
     return torch
 
 
+def _params(N, precision, L, kscale, Tau, Gamma, dt, shape):
+    return _Params(int(N), int(precision), float(L), float(kscale), float(Tau), float(Gamma), float(dt), int(shape))
+
+
+def shapes(N):
+    """FFT pass shapes of N: list of (row, j, i) threads per pencil; index 0 is the default."""
+    n = ctypes.c_int()
+    _check(lib().spm_shapes(int(N), ctypes.byref(n), None, 0), "spm_shapes")
+    out = np.zeros(3 * n.value, dtype=np.int32)
+    _check(lib().spm_shapes(int(N), ctypes.byref(n), out.ctypes.data_as(_c), n.value), "spm_shapes")
+    return [tuple(int(x) for x in out[3 * k:3 * k + 3]) for k in range(n.value)]
+
+
+def tune(N, precision=FP64, nmem=1, steps=20, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3):
+    """Time every shape of N (batch of nmem members); returns (best, us_per_step list).
+    The choice is recorded in the tuning cache read for shape=-1."""
+    _torch()
+    p = _params(N, precision, L, kscale, Tau, Gamma, dt, 0)
+    best = ctypes.c_int()
+    us = np.zeros(len(shapes(N)))
+    _check(lib().spm_tune(ctypes.byref(p), int(nmem), int(steps), ctypes.byref(best), us.ctypes.data_as(_c)),
+           "spm_tune")
+    return best.value, us.tolist()
+
+
 class Grid:
     """One device context: an N^3 grid, the model parameters and a precision.
 
@@ -129,10 +165,10 @@ class Grid:
     complex.  Mirrors the reference's process-global state (Data.h:4-28).
     """
 
-    def __init__(self, N, precision=FP64, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3):
+    def __init__(self, N, precision=FP64, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, shape=0):
         torch = _torch()
         self.torch = torch
-        p = _Params(int(N), int(precision), float(L), float(kscale), float(Tau), float(Gamma), float(dt))
+        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape)
         h = _c()
         _check(lib().spm_create(ctypes.byref(p), ctypes.byref(h)), "spm_create")
         self.h = h
@@ -365,3 +401,90 @@ class Grid:
         _check(lib().spm_lin2(self.h, self._real(r), float(a1), self._real(F1), float(a2), self._real(F2)),
                "spm_lin2")
         return r
+
+
+class Batch:
+    """A batch of independent gradient flows (get_min_u_F, LPsrc.cpp:40-117)
+    of one grid size and precision, each member with its own (Tau, Gamma) and
+    limits; members are loaded and read while the others run.  Fields are
+    torch tensors on the GPU in the batch precision."""
+
+    def __init__(self, N, precision=FP64, capacity=8, hist_cap=4096, L=26.0, kscale=1.0, Tau=-0.0061,
+                 Gamma=0.30, dt=0.3, shape=0):
+        torch = _torch()
+        self.torch = torch
+        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape)
+        h = _c()
+        _check(lib().spm_batch_create(ctypes.byref(p), int(capacity), int(hist_cap), ctypes.byref(h)),
+               "spm_batch_create")
+        self.h = h
+        self.N, self.capacity, self.hist_cap = int(N), int(capacity), int(hist_cap)
+        self.precision = int(precision)
+        self.dtype = torch.float64 if precision == FP64 else torch.float32
+        cap, nb = ctypes.c_int(), ctypes.c_size_t()
+        _check(lib().spm_batch_info(self.h, ctypes.byref(cap), ctypes.byref(nb)), "spm_batch_info")
+        self.dev_bytes = nb.value
+        s = _c()
+        _check(lib().spm_batch_stream(self.h, ctypes.byref(s)), "spm_batch_stream")
+        self.stream = torch.cuda.ExternalStream(s.value) if s.value else None
+        self._keep = {}  # inputs read asynchronously: held until the next poll / read
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spm_batch_destroy(self.h)
+            self.h = None
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _field(self, t):
+        if not t.is_cuda or t.dtype != self.dtype or not t.is_contiguous() or t.numel() != self.N ** 3:
+            raise SpmError("field must be a contiguous (N, N, N) %s GPU tensor" % self.dtype)
+        return _c(t.data_ptr())
+
+    def load(self, slot, phi, Tau, Gamma, max_iter, res_tol=1e-16, gn_tol=0.0):
+        """Start member `slot` from phi (copied into the member's state)."""
+        self.torch.cuda.current_stream().synchronize()  # phi is read on the batch stream
+        _check(lib().spm_batch_load(self.h, int(slot), self._field(phi), float(Tau), float(Gamma), int(max_iter),
+                                    float(res_tol), float(gn_tol)), "spm_batch_load")
+        self._keep[slot] = phi
+
+    def steps(self, n):
+        """Enqueue n iterations of every running member (asynchronous)."""
+        _check(lib().spm_batch_steps(self.h, int(n)), "spm_batch_steps")
+
+    def poll(self):
+        """(running count, iters array, stopped array); synchronises."""
+        it = np.zeros(self.capacity, dtype=np.int32)
+        st = np.zeros(self.capacity, dtype=np.int32)
+        rc = lib().spm_batch_poll(self.h, it.ctypes.data_as(_c), st.ctypes.data_as(_c))
+        if rc < 0:
+            _check(rc, "spm_batch_poll")
+        self._keep = {}
+        return rc, it, st
+
+    def read(self, slot, phi=None, want_hist=True):
+        """Field (into phi, or a new tensor) and history of a stopped member."""
+        if phi is None:
+            phi = self.torch.empty((self.N,) * 3, dtype=self.dtype, device="cuda")
+        hist = np.zeros((self.hist_cap, 3))
+        it = ctypes.c_int()
+        self.torch.cuda.current_stream().synchronize()  # phi is written on the batch stream
+        _check(lib().spm_batch_read(self.h, int(slot), self._field(phi),
+                                    hist.ctypes.data_as(_c) if want_hist else None, ctypes.byref(it)),
+               "spm_batch_read")
+        self._keep.pop(slot, None)
+        return phi, hist[: it.value]
+
+    def release(self, slot):
+        _check(lib().spm_batch_release(self.h, int(slot)), "spm_batch_release")

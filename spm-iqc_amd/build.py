@@ -29,7 +29,7 @@ CFLAGS = [
     "-fPIC",
     "-Wall",
     "-Wno-unused-function",
-    # packed FP32 instructions stay off (numerics)
+    # packed FP32 instructions stay off: some of their forms have given wrong results on gfx950
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(HERE, "..", "include"),
 ]

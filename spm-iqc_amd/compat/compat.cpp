@@ -100,7 +100,7 @@ spm_ctx *ctx()
         g_ctx = nullptr;
         free_scratch();
     }
-    spm_params p;
+    spm_params p{};
     p.N = N;
     p.precision = SPM_FP64;
     p.L = L;

@@ -22,6 +22,13 @@ struct spm_ctx {
 };
 
 int spm_fail(int code, const std::string &msg);
+// context of nmem batch members (1: a single field) with hist_cap iterations of history each
+int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out);
+void spm_ctx_free(spm_ctx *h);
+// control block of member m: iteration 0, the given limits (max_iter <= 0: stopped)
+void spm_ctl_init(spm_ctx *h, int m, int max_iter, double res_tol, double gn_tol);
+// shape recorded by the tuner for (p->N, p->precision, nmem), tuning on a miss
+int spm_resolve_shape(const spm_params *p, int nmem, int *shape);
 int spm_dev_alloc(spm_ctx *h, void **p, size_t bytes);
 int spm_launch_status(const char *what);
 int spm_read_scalars(spm_ctx *h, const double *dev, int n, double *out);

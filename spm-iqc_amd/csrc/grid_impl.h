@@ -24,36 +24,38 @@ struct GridImpl final : GridIface {
     }
 
     // ---------------------------------------------------------- launchers
+    // nm > 1: nm batch members (blockIdx.y), member m at in + m * in_bs, out + m * out_bs
     template <bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
     static void rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl = nullptr,
-                     Pro pro = Pro{})
+                     Pro pro = Pro{}, int nm = 1, long in_bs = 0, long out_bs = 0)
     {
         const long nb = (NPAIRS + GR::PP - 1) / GR::PP;
-        hipLaunchKernelGGL((k_rows<T, N, TR, IN_SPEC, OUT_SPEC, Op, Pro>), dim3((unsigned)nb), dim3(GR::ROW_THREADS), 0,
-                           c.stream, in, (CT *)out, c.NHP, NPAIRS, (const CT *)c.tw, op, part, ctl, pro);
+        hipLaunchKernelGGL((k_rows<T, N, TR, IN_SPEC, OUT_SPEC, Op, Pro>), dim3((unsigned)nb, (unsigned)nm),
+                           dim3(GR::ROW_THREADS), 0, c.stream, in, (CT *)out, c.NHP, NPAIRS, (const CT *)c.tw, op,
+                           part, ctl, pro, in_bs, out_bs);
     }
     // pencils along j inside each i plane; row length nhp, valid nh
     template <bool FWD, bool INV, bool STORE, class SOp, class Pro = ProNone>
     static void cols_j(Ctx &c, const void *src, void *dst, SOp op, double *part, const Ctl *ctl = nullptr,
-                       int nh = -1, int nhp = -1, Pro pro = Pro{})
+                       int nh = -1, int nhp = -1, Pro pro = Pro{}, int nm = 1, long bs = 0)
     {
         if (nh < 0) { nh = c.NH; nhp = c.NHP; }
         const long nb = (long)N * ((nhp + GJ::B - 1) / GJ::B);
-        hipLaunchKernelGGL((k_cols<T, N, TJ, FWD, INV, STORE, SOp, Pro>), dim3((unsigned)nb), dim3(GJ::COL_THREADS), 0,
-                           c.stream, (const CT *)src, (CT *)dst, (long)nhp, (long)nhp, (long)N * nhp, nh, nhp, 0,
-                           (const CT *)c.tw, op, part, ctl, pro);
+        hipLaunchKernelGGL((k_cols<T, N, TJ, FWD, INV, STORE, SOp, Pro>), dim3((unsigned)nb, (unsigned)nm),
+                           dim3(GJ::COL_THREADS), 0, c.stream, (const CT *)src, (CT *)dst, (long)nhp, (long)nhp,
+                           (long)N * nhp, nh, nhp, 0, (const CT *)c.tw, op, part, ctl, pro, bs);
     }
     // pencils along i over the whole array
     template <bool FWD, bool INV, bool STORE, class SOp>
     static void cols_i(Ctx &c, const void *src, void *dst, SOp op, double *part, const Ctl *ctl = nullptr,
-                       int nh = -1, int nhp = -1)
+                       int nh = -1, int nhp = -1, int nm = 1, long bs = 0)
     {
         if (nh < 0) { nh = c.NH; nhp = c.NHP; }
         const long ncols = (long)N * nhp;
         const long nb = (ncols + GI::B - 1) / GI::B;
-        hipLaunchKernelGGL((k_cols<T, N, TI, FWD, INV, STORE, SOp>), dim3((unsigned)nb), dim3(GI::COL_THREADS), 0,
-                           c.stream, (const CT *)src, (CT *)dst, ncols, ncols, 0L, nh, nhp, 1, (const CT *)c.tw, op,
-                           part, ctl);
+        hipLaunchKernelGGL((k_cols<T, N, TI, FWD, INV, STORE, SOp, ProNone>), dim3((unsigned)nb, (unsigned)nm),
+                           dim3(GI::COL_THREADS), 0, c.stream, (const CT *)src, (CT *)dst, ncols, ncols, 0L, nh, nhp,
+                           1, (const CT *)c.tw, op, part, ctl, ProNone{}, bs);
     }
 
     // ---------------------------------------------------------- transforms
@@ -64,11 +66,13 @@ struct GridImpl final : GridIface {
         cols_i<true, false, true>(c, S, S, SOpScale<T>{1.0 / (double)c.R}, nullptr);
     }
 
-    void c2r(Ctx &c, const void *S, void *x) override
+    void c2r(Ctx &c, const void *S, void *x) override { c2r_ws(c, S, x, c.W); }
+
+    static void c2r_ws(Ctx &c, const void *S, void *x, void *ws)
     {
-        cols_i<false, true, true>(c, S, c.W, SOpNone{}, nullptr);
-        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
-        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+        cols_i<false, true, true>(c, S, ws, SOpNone{}, nullptr);
+        cols_j<false, true, true>(c, ws, ws, SOpNone{}, nullptr);
+        rows<true, false>(c, ws, nullptr, OpStore<T>{(T *)x}, nullptr);
     }
 
     void c2c(Ctx &c, const void *in, void *out, int dir) override
@@ -151,27 +155,43 @@ struct GridImpl final : GridIface {
     // State: U = u_F (normalised spectrum of phi).  W carries the spectrum of
     // f(phi) between passes.  Slot 2 holds the E2 sums of the current phi
     // (row pass), slot 3 the E1 / ||grad F||^2 / residual sums (i pass).
-    void gf_init(Ctx &c, const void *phi) override
+    void gf_init(Ctx &c, const void *phi) override { gf_init_member(c, 0, phi, c.Tau, c.Gamma); }
+
+    // Member m of a batch (m = 0 for a single field): its U and W are the m-th
+    // spectra, its partial sums the m-th share of each slot, its control block ctl[m].
+    void gf_init_member(Ctx &c, int m, const void *phi, double Tau, double Gamma) override
     {
-        r2c(c, phi, c.U);
-        rows<false, true>(c, phi, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr}, c.slot(2), c.ctl);
-        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
+        CT *Um = (CT *)c.U + (long)m * c.C, *Wm = (CT *)c.W + (long)m * c.C;
+        r2c(c, phi, Um);
+        rows<false, true>(c, phi, Wm, OpGradFlowRow<T>{Tau, Gamma, nullptr, nullptr, 0}, c.slot(2) + (long)m * c.nb_rows,
+                          c.ctl + m);
+        cols_j<true, false, true>(c, Wm, Wm, SOpNone{}, nullptr, c.ctl + m);
+    }
+
+    // phi of member m from its state spectrum (its W is the scratch)
+    void gf_read_member(Ctx &c, int m, void *phi) override
+    {
+        c2r_ws(c, (const CT *)c.U + (long)m * c.C, phi, (CT *)c.W + (long)m * c.C);
     }
 
     // One get_min_u_F iteration (LPsrc.cpp:52-111).  Kernels return at once
     // when ctl->stop is set, so a fixed sequence of steps can be queued (or
     // replayed from a graph) past convergence.
+    // Batched (c.nmem members): every launch covers all members, each with its
+    // own (Tau, Gamma) from c.prm; members whose ctl[m].stop is set skip out.
     void gf_step(Ctx &c) override
     {
         const int kf = (int)std::min<long>(64, c.nb_j);
+        const int nm = c.nmem;
+        const long bs = c.C;
         cols_i<true, true, true>(c, c.W, c.W,
-                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U},
-                                 c.slot(3), c.ctl);
+                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs},
+                                 c.slot(3), c.ctl, -1, -1, nm, bs);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl, -1, -1,
-                                  ProGradFlowReduce{c.slot(3), c.nb_i, c.slot(2), c.nb_rows, c.slot(4), kf});
-        rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr}, c.slot(2), c.ctl,
-                         ProGradFlowFinalize{c.slot(4), kf, (double)c.R, c.ctl, c.hist});
-        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl);
+                                  ProGradFlowReduce{c.slot(3), c.nb_i, c.slot(2), c.nb_rows, c.slot(4), kf}, nm, bs);
+        rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr, c.prm, 0}, c.slot(2), c.ctl,
+                         ProGradFlowFinalize{c.slot(4), kf, (double)c.R, c.ctl, c.hist, 3 * c.hist_cap}, nm, bs, bs);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl, -1, -1, ProNone{}, nm, bs);
     }
 };
 

@@ -1,27 +1,28 @@
-// Instantiation of the N = 256 passes, with the threads-per-pencil variants
-// the tuner chooses from (one translation unit per N so they compile in parallel).
-#include <cstdlib>
-
+// Instantiation of the N = 256 passes and their tunable shapes (threads per
+// pencil of the row, j and i passes; one translation unit per N so they
+// compile in parallel).
 #include "grid_impl.h"
 
 namespace spm {
-GridIface *make_grid_n256(int prec)
+namespace {
+template <typename T> static GridIface *make(int shape)
 {
-    int v = 0;
-    if (const char *e = std::getenv("SPM_VARIANT_256")) v = std::atoi(e);
-    if (prec) {
-        switch (v) {
-        case 1: return new GridImpl<float, 256, 32, 16, 32>();
-        case 2: return new GridImpl<float, 256, 16, 16, 32>();
-        case 3: return new GridImpl<float, 256, 32, 16, 16>();
-        default: return new GridImpl<float, 256>();
-        }
+    switch (shape) {
+    case 1: return new GridImpl<T, 256, 32, 16, 32>();
+    case 2: return new GridImpl<T, 256, 16, 16, 32>();
+    case 3: return new GridImpl<T, 256, 32, 16, 16>();
+    default: return new GridImpl<T, 256>();
     }
-    switch (v) {
-    case 1: return new GridImpl<double, 256, 32, 16, 32>();
-    case 2: return new GridImpl<double, 256, 16, 16, 32>();
-    case 3: return new GridImpl<double, 256, 32, 16, 16>();
-    default: return new GridImpl<double, 256>();
-    }
+}
+} // namespace
+
+const int kShapes256[][3] = {{16, 16, 16}, {32, 16, 32}, {16, 16, 32}, {32, 16, 16}};
+
+GridIface *make_grid_n256(int prec, int shape) { return prec ? make<float>(shape) : make<double>(shape); }
+
+int grid_shapes_n256(const int (**s)[3])
+{
+    *s = kShapes256;
+    return (int)(sizeof(kShapes256) / sizeof(kShapes256[0]));
 }
 } // namespace spm

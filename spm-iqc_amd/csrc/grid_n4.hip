@@ -1,10 +1,19 @@
-// Instantiation of the N = 4 passes (one translation unit per N so they compile in parallel).
+// Instantiation of the N = 4 passes (one translation unit per N so they
+// compile in parallel).
 #include "grid_impl.h"
 
 namespace spm {
-GridIface *make_grid_n4(int prec)
+namespace {
+template <typename T> static GridIface *make(int) { return new GridImpl<T, 4>(); }
+} // namespace
+
+const int kShapes4[][3] = {{1, 1, 1}};
+
+GridIface *make_grid_n4(int prec, int shape) { return prec ? make<float>(shape) : make<double>(shape); }
+
+int grid_shapes_n4(const int (**s)[3])
 {
-    if (prec) return new GridImpl<float, 4>();
-    return new GridImpl<double, 4>();
+    *s = kShapes4;
+    return (int)(sizeof(kShapes4) / sizeof(kShapes4[0]));
 }
 } // namespace spm

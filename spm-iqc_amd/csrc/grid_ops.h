@@ -20,6 +20,8 @@
 //           and the inverse transform, so the spectrum of a step never makes
 //           an extra trip through HBM.
 #pragma once
+#include <utility>
+
 #include "fft_core.h"
 
 namespace spm {
@@ -49,6 +51,21 @@ template <typename T, int N, int TPP_ = DefTPP<N>::value> struct Geo {
 };
 
 __device__ __forceinline__ int kfold(int i, int N) { return i < N / 2 ? i : i - N; }
+
+// Batched launches: blockIdx.y selects the member of a batch of independent
+// fields stored one after another.  An operator with member-dependent state
+// (parameters, state arrays, partial-sum slots) has bind(m), which returns
+// its copy for member m; operators without one are shared by all members.
+template <class Op, class = void> struct has_bind : std::false_type {};
+template <class Op>
+struct has_bind<Op, std::void_t<decltype(std::declval<const Op &>().bind(0))>> : std::true_type {};
+template <class Op> __device__ __forceinline__ Op bind_member(const Op &op, int m)
+{
+    if constexpr (has_bind<Op>::value)
+        return op.bind(m);
+    else
+        return op;
+}
 
 // ======================================================================
 // Real-space operators applied to each real value inside the row pass.
@@ -80,6 +97,18 @@ template <typename T> struct OpGradFlowRow {
     static constexpr unsigned MAXM = 0;
     double Tau, Gamma;
     T *phi_out;
+    const double *prm; // batched: (Tau, Gamma) of member m at prm[2m], prm[2m+1]
+    long out_bs;       // batched: member stride of phi_out
+    __device__ __forceinline__ OpGradFlowRow bind(int m) const
+    {
+        OpGradFlowRow o = *this;
+        if (prm) {
+            o.Tau = prm[2 * m];
+            o.Gamma = prm[2 * m + 1];
+        }
+        if (phi_out) o.phi_out = phi_out + (long)m * out_bs;
+        return o;
+    }
     __device__ __forceinline__ T operator()(T x, long g, double *acc) const
     {
         const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
@@ -263,6 +292,13 @@ template <typename T, int N> struct SOpGradFlowStep {
     double sc, dt;
     const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
+    long ubs; // batched: member stride of U
+    __device__ __forceinline__ SOpGradFlowStep bind(int m) const
+    {
+        SOpGradFlowStep o = *this;
+        o.U = U + (long)m * ubs;
+        return o;
+    }
     __device__ __forceinline__ const C2<T> *pre_src() const { return U; }
     __device__ __forceinline__ void operator()(C2<T> &v, C2<T> u, long s, int i, int j, int kz, double *acc) const
     {
@@ -312,6 +348,14 @@ struct ProGradFlowReduce {
     long nC;
     double *p2;
     int kf;
+    __device__ __forceinline__ ProGradFlowReduce bind(int m) const
+    {
+        ProGradFlowReduce o = *this;
+        o.pA = pA + (long)m * 3 * nA;
+        o.pC = pC + (long)m * nC;
+        o.p2 = p2 + (long)m * 4 * kf;
+        return o;
+    }
     __device__ int nblk() const { return kf; }
     __device__ void operator()(double *red) const
     {
@@ -340,6 +384,15 @@ struct ProGradFlowFinalize {
     double R;
     Ctl *ctl;
     double *hist;
+    long hstride; // batched: member stride of hist (doubles)
+    __device__ __forceinline__ ProGradFlowFinalize bind(int m) const
+    {
+        ProGradFlowFinalize o = *this;
+        o.p2 = p2 + (long)m * 4 * kf;
+        o.ctl = ctl + m;
+        o.hist = hist + (long)m * hstride;
+        return o;
+    }
     __device__ int nblk() const { return 1; }
     __device__ void operator()(double *red) const
     {
@@ -376,7 +429,8 @@ __device__ __forceinline__ void write_partials(double (&acc)[NRED > 0 ? NRED : 1
         block_reduce<NRED, MAXM>(acc, red);
         if (threadIdx.x == 0)
 #pragma unroll
-            for (int q = 0; q < NRED; ++q) part[(long)q * gridDim.x + blockIdx.x] = acc[q];
+            for (int q = 0; q < NRED; ++q)
+                part[((long)blockIdx.y * NRED + q) * gridDim.x + blockIdx.x] = acc[q];
     }
 }
 
@@ -396,8 +450,8 @@ __device__ __forceinline__ void load_twiddles(C2<T> *stw, const C2<T> *__restric
 // LDS with coalesced loads, each element read once.
 template <typename T, int N, int TPP_, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
 __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
-    k_rows(const void *in, C2<T> *out, int NHP, long npairs, const C2<T> *__restrict__ tw, Op op,
-           double *__restrict__ part, const Ctl *ctl, Pro pro = Pro{})
+    k_rows(const void *in, C2<T> *out, int NHP, long npairs, const C2<T> *__restrict__ tw, Op op0,
+           double *__restrict__ part, const Ctl *ctl, Pro pro0, long in_bs, long out_bs)
 {
     using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, PP = G::PP, NH = N / 2 + 1;
@@ -405,9 +459,18 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
     __shared__ C2<T> lds[XCH > STG ? XCH : STG];
     __shared__ C2<T> stw[N];
     __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
-    if (ctl && ctl->stop) return;
-    if constexpr (!Pro::NONE)
+    const int mb = blockIdx.y; // batch member
+    if (ctl && ctl[mb].stop) return;
+    const Op op = bind_member(op0, mb);
+    if constexpr (!Pro::NONE) {
+        const Pro pro = bind_member(pro0, mb);
         if ((int)blockIdx.x < pro.nblk()) pro(red);
+    }
+    if constexpr (IN_SPEC)
+        in = (const C2<T> *)in + mb * in_bs;
+    else
+        in = (const T *)in + mb * in_bs;
+    if (out) out += mb * out_bs;
     load_twiddles<T, N>(stw, tw);
     const int t = threadIdx.x % TPP, pp = threadIdx.x / TPP;
     const long pair = (long)blockIdx.x * PP + pp;
@@ -523,17 +586,23 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
 template <typename T, int N, int TPP_, bool FWD, bool INV, bool STORE, class SOp, class Pro = ProNone>
 __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
     k_cols(const C2<T> *src, C2<T> *dst, long pstride, long ncols, long ostride, int NH, int NHP,
-           int iaxis, const C2<T> *__restrict__ tw, SOp op, double *__restrict__ part,
-           const Ctl *ctl, Pro pro = Pro{})
+           int iaxis, const C2<T> *__restrict__ tw, SOp op0, double *__restrict__ part,
+           const Ctl *ctl, Pro pro0, long bs)
 {
     using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, B = G::B;
     __shared__ C2<T> lds[N * B];
     __shared__ C2<T> stw[N];
     __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
-    if (ctl && ctl->stop) return;
-    if constexpr (!Pro::NONE)
+    const int mb = blockIdx.y; // batch member
+    if (ctl && ctl[mb].stop) return;
+    const SOp op = bind_member(op0, mb);
+    if constexpr (!Pro::NONE) {
+        const Pro pro = bind_member(pro0, mb);
         if ((int)blockIdx.x < pro.nblk()) pro(red);
+    }
+    src += mb * bs;
+    if (dst) dst += mb * bs;
     load_twiddles<T, N>(stw, tw);
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const long tpo = (ncols + B - 1) / B;

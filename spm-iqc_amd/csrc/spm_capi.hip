@@ -94,28 +94,31 @@ void launch_reduce(Ctx &c, const double *part, long nblk, int nval, unsigned max
     hipLaunchKernelGGL(k_reduce, dim3(nval), dim3(256), 0, c.stream, part, nblk, maxmask, scale, out);
 }
 
-GridIface *make_grid_n4(int), *make_grid_n8(int), *make_grid_n12(int), *make_grid_n16(int),
-    *make_grid_n24(int), *make_grid_n32(int), *make_grid_n48(int), *make_grid_n64(int), *make_grid_n96(int),
-    *make_grid_n128(int), *make_grid_n192(int), *make_grid_n256(int);
+#define SPM_FOR_EACH_N(X) X(4) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+#define SPM_DECL_N(n) GridIface *make_grid_n##n(int, int); int grid_shapes_n##n(const int (**)[3]);
+SPM_FOR_EACH_N(SPM_DECL_N)
+#undef SPM_DECL_N
 
-GridIface *make_grid(int N, int prec)
+int grid_shapes(int N, const int (**s)[3])
 {
     switch (N) {
-    case 4: return make_grid_n4(prec);
-    case 8: return make_grid_n8(prec);
-    case 12: return make_grid_n12(prec);
-    case 16: return make_grid_n16(prec);
-    case 24: return make_grid_n24(prec);
-    case 32: return make_grid_n32(prec);
-    case 48: return make_grid_n48(prec);
-    case 64: return make_grid_n64(prec);
-    case 96: return make_grid_n96(prec);
-    case 128: return make_grid_n128(prec);
-    case 192: return make_grid_n192(prec);
-    case 256: return make_grid_n256(prec);
+#define SPM_CASE_N(n) case n: return grid_shapes_n##n(s);
+        SPM_FOR_EACH_N(SPM_CASE_N)
+#undef SPM_CASE_N
+    default: return 0;
+    }
+}
+
+GridIface *make_grid(int N, int prec, int shape)
+{
+    switch (N) {
+#define SPM_CASE_N(n) case n: return make_grid_n##n(prec, shape);
+        SPM_FOR_EACH_N(SPM_CASE_N)
+#undef SPM_CASE_N
     default: return nullptr;
     }
 }
+#undef SPM_FOR_EACH_N
 
 } // namespace spm
 
@@ -148,7 +151,7 @@ int spm_dev_alloc(spm_ctx *h, void **p, size_t bytes)
 static void free_all(spm_ctx *h)
 {
     Ctx &c = h->c;
-    void *ps[] = {c.tw, c.Gtab, c.Itab, c.Btab, c.GDtab, c.W, c.U, c.part, c.scal, c.ctl, c.hist, c.counter};
+    void *ps[] = {c.tw, c.Gtab, c.Itab, c.Btab, c.GDtab, c.W, c.U, c.part, c.scal, c.ctl, c.hist, c.counter, c.prm};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->h_pin) (void)hipHostFree(h->h_pin);
@@ -195,7 +198,16 @@ const char *spm_last_error(void) { return g_err.c_str(); }
 int spm_create(const spm_params *p, spm_ctx **out)
 {
     if (!p || !out) return spm_fail(SPM_EINVAL, "spm_create: null argument");
+    return spm_ctx_create(p, 1, 1024, out);
+}
+
+} // extern "C"
+
+int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
+{
+    if (!p || !out) return spm_fail(SPM_EINVAL, "spm_create: null argument");
     *out = nullptr;
+    if (nmem < 1) return spm_fail(SPM_EINVAL, "spm_create: batch capacity must be >= 1");
     if (!supported_N(p->N)) return spm_fail(SPM_ENOTSUP, "spm_create: unsupported N " + std::to_string(p->N));
     if (p->precision != SPM_FP64 && p->precision != SPM_FP32)
         return spm_fail(SPM_EINVAL, "spm_create: precision must be SPM_FP64 or SPM_FP32");
@@ -216,7 +228,23 @@ int spm_create(const spm_params *p, spm_ctx **out)
     c.Tau = p->Tau;
     c.Gamma = p->Gamma;
     c.dt = p->dt;
-    c.g = make_grid(c.N, c.prec);
+    int shape = p->shape;
+    if (shape < 0) { // the tuned shape for this size, precision and batch (measured on a miss)
+        const int rc = spm_resolve_shape(p, nmem, &shape);
+        if (rc) {
+            delete h;
+            return rc;
+        }
+    }
+    const int (*shapes)[3] = nullptr;
+    if (shape >= grid_shapes(c.N, &shapes)) {
+        delete h;
+        return spm_fail(SPM_EINVAL, "spm_create: no FFT pass shape " + std::to_string(shape) + " for N " +
+                                        std::to_string(c.N));
+    }
+    c.shape = shape;
+    c.nmem = nmem;
+    c.g = make_grid(c.N, c.prec, shape);
     if (!c.g) {
         delete h;
         return spm_fail(SPM_ENOTSUP, "spm_create: no kernels for this N");
@@ -277,32 +305,63 @@ int spm_create(const spm_params *p, spm_ctx **out)
             return bail(spm_fail(SPM_EHIP, "spm_create: table upload failed"));
     }
     const size_t spec = (size_t)c.C * 2 * c.esz;
-    if ((rc = spm_dev_alloc(h, &c.W, spec))) return bail(rc);
-    if ((rc = spm_dev_alloc(h, &c.U, spec))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, &c.W, spec * nmem))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, &c.U, spec * nmem))) return bail(rc);
     long mb = std::max(std::max(c.nb_rows, c.nb_j), std::max(c.nb_i, (long)DOT_BLOCKS));
-    c.part_slot = 4 * mb;
+    c.part_slot = 4 * mb * nmem; // every member's share of a gradient-flow slot
     c.nslots = 8;
     if ((rc = spm_dev_alloc(h, (void **)&c.part, sizeof(double) * c.part_slot * c.nslots))) return bail(rc);
     if ((rc = spm_dev_alloc(h, (void **)&c.scal, sizeof(double) * SC_COUNT))) return bail(rc);
-    if ((rc = spm_dev_alloc(h, (void **)&c.ctl, sizeof(Ctl)))) return bail(rc);
+    if ((rc = spm_dev_alloc(h, (void **)&c.ctl, sizeof(Ctl) * nmem))) return bail(rc);
+    if (nmem > 1) { // every member starts with the context's (Tau, Gamma)
+        if ((rc = spm_dev_alloc(h, (void **)&c.prm, sizeof(double) * 2 * nmem))) return bail(rc);
+        std::vector<double> prm(2 * nmem);
+        for (int m = 0; m < nmem; ++m) {
+            prm[2 * m] = c.Tau;
+            prm[2 * m + 1] = c.Gamma;
+        }
+        if (hipMemcpy(c.prm, prm.data(), sizeof(double) * 2 * nmem, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(spm_fail(SPM_EHIP, "spm_create: parameter upload failed"));
+    }
     if ((rc = spm_dev_alloc(h, (void **)&c.counter, 64 * sizeof(unsigned)))) return bail(rc);
     if (hipMemset(c.counter, 0, 64 * sizeof(unsigned)) != hipSuccess)
         return bail(spm_fail(SPM_EHIP, "spm_create: memset"));
     if (hipHostMalloc((void **)&h->h_pin, 4096, hipHostMallocDefault) != hipSuccess)
         return bail(spm_fail(SPM_ENOMEM, "spm_create: pinned scratch"));
-    if ((rc = ensure_hist(h, 1024))) return bail(rc);
+    if (nmem == 1) {
+        if ((rc = ensure_hist(h, hist_cap))) return bail(rc);
+    } else { // fixed: a captured step graph holds the address
+        if ((rc = spm_dev_alloc(h, (void **)&c.hist, sizeof(double) * 3 * hist_cap * nmem))) return bail(rc);
+        c.hist_cap = hist_cap;
+    }
     if (hipMemset(c.scal, 0, sizeof(double) * SC_COUNT) != hipSuccess)
         return bail(spm_fail(SPM_EHIP, "spm_create: memset"));
+    for (int m = 0; m < nmem; ++m) // every member starts stopped
+        hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, c.stream, c.ctl + m, 0, 0.0, 0.0);
+    if ((rc = spm_launch_status("spm_create"))) return bail(rc);
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return bail(spm_fail(SPM_EHIP, "spm_create: sync"));
     *out = h;
     return SPM_OK;
 }
 
-int spm_destroy(spm_ctx *h)
+void spm_ctl_init(spm_ctx *h, int m, int max_iter, double res_tol, double gn_tol)
 {
-    if (!h) return SPM_OK;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, h->c.stream, h->c.ctl + m, max_iter, res_tol, gn_tol);
+}
+
+void spm_ctx_free(spm_ctx *h)
+{
+    if (!h) return;
     (void)hipStreamSynchronize(h->c.stream);
     free_all(h);
     delete h;
+}
+
+extern "C" {
+
+int spm_destroy(spm_ctx *h)
+{
+    spm_ctx_free(h);
     return SPM_OK;
 }
 

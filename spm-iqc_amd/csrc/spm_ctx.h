@@ -48,6 +48,12 @@ struct Ctx {
     long hist_cap = 0;
     // block counts of the passes over one field
     long nb_rows = 0, nb_j = 0, nb_i = 0;
+    // batches of independent gradient flows: nmem members, member m's U and W
+    // at U + m C, W + m C, (Tau, Gamma) at prm[2m] (nullptr: the context's),
+    // control block ctl[m], history hist + 3 m hist_cap
+    int nmem = 1;
+    double *prm = nullptr;
+    int shape = 0; // FFT pass shape (index into grid_shapes(N))
 
     double *slot(int s) { return part + (long)s * part_slot; }
 };
@@ -72,10 +78,14 @@ struct GridIface {
     // (one force step of SPM.cpp:34-99 / 103-168; s a device scalar)
     virtual void force(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;
     virtual void gf_init(Ctx &c, const void *phi) = 0;
+    virtual void gf_init_member(Ctx &c, int m, const void *phi, double Tau, double Gamma) = 0;
+    virtual void gf_read_member(Ctx &c, int m, void *phi) = 0;
     virtual void gf_step(Ctx &c) = 0;
 };
 
-GridIface *make_grid(int N, int prec);
+GridIface *make_grid(int N, int prec, int shape);
+// FFT pass shapes of N: *s = table of (row, j, i) threads per pencil; returns the count
+int grid_shapes(int N, const int (**s)[3]);
 
 // generic helpers (spm_capi.hip)
 void launch_reduce(Ctx &c, const double *part, long nblk, int nval, unsigned maxmask, double scale, double *out);

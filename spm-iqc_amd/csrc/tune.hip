@@ -1,0 +1,186 @@
+// tune.hip -- choice of the FFT pass shapes (threads per pencil of the row,
+// j and i passes) by measurement, with a cache of the results on disk: the
+// counterpart of the reference's FFTW_PATIENT planning and wisdom files
+// (My_mpi_fft_set_up, My_FFT.cpp:45-97).
+//
+// Cache: one line per measurement, "arch N precision members shape us_per_step",
+// appended to $SPM_TUNE_CACHE (default $HOME/.cache/spm-iqc/shapes.txt); the
+// last line for a key wins.  Members are bucketed to the next power of two.
+#include <hip/hip_runtime.h>
+#include <sys/stat.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+
+using namespace spm;
+
+namespace {
+std::mutex g_mu;
+std::map<std::string, int> g_table;
+bool g_loaded = false;
+
+std::string cache_path()
+{
+    if (const char *e = std::getenv("SPM_TUNE_CACHE")) return e;
+    const char *home = std::getenv("HOME");
+    return std::string(home ? home : ".") + "/.cache/spm-iqc/shapes.txt";
+}
+
+std::string arch()
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    const size_t k = a.find(':');
+    return k == std::string::npos ? a : a.substr(0, k);
+}
+
+int bucket(int nmem)
+{
+    int b = 1;
+    while (b < nmem) b *= 2;
+    return b;
+}
+
+std::string key(const std::string &ar, int N, int prec, int nmem)
+{
+    return ar + " " + std::to_string(N) + " " + std::to_string(prec) + " " + std::to_string(bucket(nmem));
+}
+
+void load_cache() // g_mu held
+{
+    if (g_loaded) return;
+    g_loaded = true;
+    FILE *f = std::fopen(cache_path().c_str(), "r");
+    if (!f) return;
+    char ar[64];
+    int N, prec, nm, shape;
+    double us;
+    while (std::fscanf(f, "%63s %d %d %d %d %lf", ar, &N, &prec, &nm, &shape, &us) == 6)
+        g_table[key(ar, N, prec, nm)] = shape;
+    std::fclose(f);
+}
+
+void record(const std::string &ar, int N, int prec, int nmem, int shape, double us) // g_mu held
+{
+    g_table[key(ar, N, prec, nmem)] = shape;
+    const std::string path = cache_path();
+    for (size_t k = path.find('/', 1); k != std::string::npos; k = path.find('/', k + 1))
+        (void)mkdir(path.substr(0, k).c_str(), 0755); // mkdir -p of the parent; fopen reports failure
+    if (FILE *f = std::fopen(path.c_str(), "a")) {
+        std::fprintf(f, "%s %d %d %d %d %.3f\n", ar.c_str(), N, prec, bucket(nmem), shape, us);
+        std::fclose(f);
+    }
+}
+
+// Time `steps` iterations of a batch of nmem zero fields with shape v,
+// replayed from one captured graph (after one untimed replay).
+int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
+{
+    spm_params q = *p;
+    q.shape = v;
+    spm_ctx *h = nullptr;
+    RCHK(spm_ctx_create(&q, nmem, 2L * steps + 2, &h));
+    Ctx &c = h->c;
+    int rc = SPM_OK;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto fail = [&](const char *what, hipError_t e) {
+        rc = spm_fail(SPM_EHIP, std::string("spm_tune: ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMemsetAsync(c.U, 0, (size_t)c.C * 2 * c.esz * nmem, c.stream)) != hipSuccess) fail("memset", e);
+    if (!rc && (e = hipMemsetAsync(c.W, 0, (size_t)c.C * 2 * c.esz * nmem, c.stream)) != hipSuccess)
+        fail("memset", e);
+    for (int m = 0; !rc && m < nmem; ++m) spm_ctl_init(h, m, 2 * steps + 2, -1.0, 0.0); // never stops early
+    if (!rc && (e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        fail("capture", e);
+    if (!rc) {
+        for (int s = 0; s < steps; ++s) c.g->gf_step(c);
+        if ((e = hipStreamEndCapture(c.stream, &g)) != hipSuccess) fail("capture", e);
+    }
+    if (!rc && (e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0)) != hipSuccess) fail("instantiate", e);
+    if (!rc && ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess))
+        fail("event", e);
+    if (!rc && (e = hipGraphLaunch(ge, c.stream)) != hipSuccess) fail("launch", e);
+    if (!rc && (e = hipEventRecord(e0, c.stream)) != hipSuccess) fail("event", e);
+    if (!rc && (e = hipGraphLaunch(ge, c.stream)) != hipSuccess) fail("launch", e);
+    if (!rc && (e = hipEventRecord(e1, c.stream)) != hipSuccess) fail("event", e);
+    if (!rc && (e = hipEventSynchronize(e1)) != hipSuccess) fail("sync", e);
+    float ms = 0.f;
+    if (!rc && (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) fail("elapsed", e);
+    if (!rc) *us = 1000.0 * ms / steps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    spm_ctx_free(h);
+    return rc;
+}
+} // namespace
+
+int spm_resolve_shape(const spm_params *p, int nmem, int *shape)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        load_cache();
+        auto it = g_table.find(key(arch(), p->N, p->precision, nmem));
+        if (it != g_table.end()) {
+            *shape = it->second;
+            return SPM_OK;
+        }
+    }
+    return spm_tune(p, nmem, 20, shape, nullptr);
+}
+
+extern "C" {
+
+int spm_shapes(int N, int *count, int *tpp, int max_shapes)
+{
+    if (!count) return spm_fail(SPM_EINVAL, "spm_shapes: null count");
+    const int (*s)[3] = nullptr;
+    const int n = grid_shapes(N, &s);
+    if (n <= 0) return spm_fail(SPM_ENOTSUP, "spm_shapes: unsupported N " + std::to_string(N));
+    *count = n;
+    if (tpp)
+        for (int k = 0; k < n && k < max_shapes; ++k)
+            for (int a = 0; a < 3; ++a) tpp[3 * k + a] = s[k][a];
+    return SPM_OK;
+}
+
+int spm_tune(const spm_params *p, int nmem, int steps, int *best, double *us_per_step)
+{
+    if (!p || !best) return spm_fail(SPM_EINVAL, "spm_tune: null argument");
+    if (nmem < 1 || steps < 1) return spm_fail(SPM_EINVAL, "spm_tune: nmem >= 1, steps >= 1");
+    const int (*s)[3] = nullptr;
+    const int n = grid_shapes(p->N, &s);
+    if (n <= 0) return spm_fail(SPM_ENOTSUP, "spm_tune: unsupported N " + std::to_string(p->N));
+    int bi = -1;
+    double bt = 0.0;
+    for (int v = 0; v < n; ++v) {
+        double us = 0.0;
+        const int rc = time_shape(p, nmem, steps, v, &us);
+        if (rc) return rc;
+        if (us_per_step) us_per_step[v] = us;
+        if (bi < 0 || us < bt) {
+            bi = v;
+            bt = us;
+        }
+    }
+    *best = bi;
+    std::lock_guard<std::mutex> lk(g_mu);
+    load_cache();
+    record(arch(), p->N, p->precision, nmem, bi, bt);
+    return SPM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,243 @@
+"""The reference's stage-1 workflow (main_SPM.cpp:15-163) on device-resident
+fields: relax to a minimum (get_min_u_F), lowest Hessian modes
+(hiosd_lobpcg_initialization + repeated hiosd_lobpcg_inip), then Spring_Pairing
+from two replicas offset along the first mode.
+
+Two precisions:
+  "fp64"   every field, FFT and product in FP64 (the reference's arithmetic).
+  "mixed"  fields, FFTs and Hessian-vector products in FP32; energies, norms,
+           Gram matrices, the Cholesky-QR, the Rayleigh-Ritz problem and the
+           spring distance in FP64 (the FP32 context does those in FP64);
+           then an FP64 polish of each stage: the last `polish` gradient-flow
+           steps, `modes_polish` LOBPCG calls and the last `spm_polish` SPM
+           iterations run in an FP64 context from the FP32 result.
+
+Both runs use the same iteration budgets, so at equal step counts their
+results differ only by what the FP32 stages leave behind.
+
+    python spm-iqc_amd/pipeline.py --N 256 [--precision both] [--out report.json]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+spm = importlib.import_module("spm-iqc_amd")
+seeds = importlib.import_module("spm-iqc_amd.seeds")
+
+DEFAULTS = dict(L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3)
+
+
+def _log(verbose, *a):
+    if verbose:
+        print(*a, flush=True)
+
+
+def _gradflow(g, phi, n, gn_tol, chunk, verbose, tag):
+    """Up to n get_min_u_F iterations on phi (in place), stopping at
+    ||grad F|| <= gn_tol.  The residual test of LPsrc.cpp:52 is kept off
+    (res_tol < 0) so the budget alone bounds the run.  Returns (iters, last
+    (E, ||grad F||, res) row or None)."""
+    done, last = 0, None
+    while done < n:
+        m = min(chunk, n - done)
+        it, h = g.gradflow(phi, m, -1.0, gn_tol, check_every=min(m, 256))
+        done += it
+        if it:
+            last = h[-1]
+            _log(verbose, "  %s relax %6d  E %.15e  |grad F| %.3e" % (tag, done, last[0], last[1]))
+        if it < m:
+            break
+    return done, last
+
+
+def relax(N, phi0, precision="fp64", gn_tol=1e-8, max_iter=100000, polish=2000, chunk=5000, params=None,
+          verbose=False):
+    """Gradient flow from phi0 until ||grad F|| <= gn_tol or max_iter steps
+    (the reference's cap, LPsrc.cpp:52).  Mixed: FP32 steps for
+    max_iter - polish steps (or until the tolerance), then FP64 steps until the
+    tolerance or the budget.  Returns (phi FP64 device tensor, info)."""
+    torch = spm._torch()
+    params = dict(DEFAULTS, **(params or {}))
+    info = {"precision": precision}
+    t0 = time.perf_counter()
+    n32 = 0
+    if precision == "mixed":
+        with spm.Grid(N, spm.FP32, **params) as g32:
+            p = torch.as_tensor(np.asarray(phi0)).to(torch.float32).cuda()
+            n32, last = _gradflow(g32, p, max(max_iter - polish, 0), gn_tol, chunk, verbose, "fp32")
+            torch.cuda.current_stream().synchronize()
+        phi = p.double()
+        del p
+    else:
+        phi = torch.as_tensor(np.asarray(phi0)).to(torch.float64).cuda()
+    t1 = time.perf_counter()
+    with spm.Grid(N, spm.FP64, **params) as g:
+        n64, _ = _gradflow(g, phi, max_iter - n32, gn_tol, chunk, verbose, "fp64")
+        info["E"] = g.energy(phi)
+        info["gn"] = g.ngrad(phi)[1]
+    torch.cuda.current_stream().synchronize()
+    info.update(iters_fp32=n32, iters_fp64=n64, iters=n32 + n64, t_fp32=t1 - t0,
+                t_fp64=time.perf_counter() - t1, t=time.perf_counter() - t0)
+    return phi, info
+
+
+def lowest_modes(N, phi, precision="fp64", k=8, nwant=4, calls=10, maxiter=21, seed=0, modes_polish=1,
+                 params=None, verbose=False):
+    """hiosd_lobpcg_initialization from a seeded uniform [-1, 1) block of k
+    fields, then `calls` hiosd_lobpcg_inip calls of maxiter iterations
+    (main_SPM.cpp:81-91).  Mixed: the calls run in FP32 fields and the last
+    modes_polish of them in FP64.  Returns (ritz[:nwant], V FP64 (k, N, N, N), info)."""
+    torch = spm._torch()
+    params = dict(DEFAULTS, **(params or {}))
+    start = np.random.default_rng(seed).uniform(-1.0, 1.0, (k, N, N, N))
+    info = {"k": k, "calls": calls, "maxiter": maxiter}
+    t0 = time.perf_counter()
+    n32 = calls - modes_polish if precision == "mixed" else 0
+    ritz = None
+    if n32 > 0:
+        with spm.Grid(N, spm.FP32, **params) as g32:
+            V = g32.lobpcg_init(torch.as_tensor(start).to(torch.float32).cuda())
+            del start
+            p32 = phi.float()
+            for c in range(n32):
+                ritz = g32.lobpcg(p32, V, maxiter)
+                _log(verbose, "  fp32 lobpcg %2d  " % c + " ".join("%.12e" % x for x in ritz[:nwant]))
+            V = V.double()
+            del p32
+    with spm.Grid(N, spm.FP64, **params) as g:
+        if n32 <= 0:
+            V = g.lobpcg_init(torch.as_tensor(start).to(torch.float64).cuda())
+            del start
+        for c in range(calls - max(n32, 0)):
+            ritz = g.lobpcg(phi, V, maxiter)
+            _log(verbose, "  fp64 lobpcg %2d  " % c + " ".join("%.12e" % x for x in ritz[:nwant]))
+        info["rayleigh"] = [g.rayleighq(phi, V[i].contiguous()) for i in range(nwant)]
+    torch.cuda.current_stream().synchronize()
+    info.update(calls_fp32=max(n32, 0), calls_fp64=calls - max(n32, 0), t=time.perf_counter() - t0)
+    return np.asarray(ritz[:nwant]), V, info
+
+
+def saddle(N, phi, v, precision="fp64", offset=0.01, maxiter=30000, tol=1e-9, spm_polish=2000, chunk=1000,
+           params=None, verbose=False):
+    """Spring_Pairing (SPM.cpp:170-263) from the replicas phi -/+ offset v
+    (maxiter, tol as main_SPM.cpp:122).  Mixed: FP32 replicas for maxiter -
+    spm_polish iterations, then FP64.  Returns (phi0, phi1 FP64, info)."""
+    torch = spm._torch()
+    params = dict(DEFAULTS, **(params or {}))
+    info = {"offset": offset}
+    t0 = time.perf_counter()
+    p0 = phi - offset * v
+    p1 = phi + offset * v
+    n32, res = 0, float("nan")
+
+    def run(g, a, b, n, tag):
+        done, r = 0, float("nan")
+        while done < n:
+            m = min(chunk, n - done)
+            it, r, _ = g.spring_pairing(a, b, m, tol)
+            done += it
+            _log(verbose, "  %s spm %6d  res %.3e" % (tag, done, r))
+            if it < m or r <= tol:
+                break
+        return done, r
+
+    if precision == "mixed":
+        with spm.Grid(N, spm.FP32, **params) as g32:
+            a, b = p0.float(), p1.float()
+            n32, res = run(g32, a, b, max(maxiter - spm_polish, 0), "fp32")
+            p0, p1 = a.double(), b.double()
+            del a, b
+    t1 = time.perf_counter()
+    n64 = 0
+    with spm.Grid(N, spm.FP64, **params) as g:
+        if not (res <= tol):
+            n64, res = run(g, p0, p1, maxiter - n32, "fp64")
+        info["E"] = g.energy(p1)
+        info["gn"] = g.ngrad(p1)[1]
+        info["E_phi0"] = g.energy(p0)
+    torch.cuda.current_stream().synchronize()
+    info.update(iters_fp32=n32, iters_fp64=n64, iters=n32 + n64, res=res, t_fp32=t1 - t0,
+                t=time.perf_counter() - t0, maxabs=float(p1.abs().max().item()))
+    return p0, p1, info
+
+
+def stage1(N, phi0, precision="fp64", params=None, relax_kw=None, modes_kw=None, saddle_kw=None, verbose=False):
+    """relax -> lowest modes -> saddle (main_SPM.cpp:15-163 with the replicas
+    offset symmetrically).  Returns (report dict, device fields dict)."""
+    phi, ri = relax(N, phi0, precision, params=params, verbose=verbose, **(relax_kw or {}))
+    ritz, V, mi = lowest_modes(N, phi, precision, params=params, verbose=verbose, **(modes_kw or {}))
+    v1 = V[0].clone()
+    del V
+    p0, p1, si = saddle(N, phi, v1, precision, params=params, verbose=verbose, **(saddle_kw or {}))
+    rep = {"N": N, "precision": precision, "relax": ri, "eigenvalues": [float(x) for x in ritz], "modes": mi,
+           "saddle": si}
+    return rep, {"phi": phi, "v1": v1, "phi0": p0, "phi1": p1}
+
+
+def compare(ref, rep, fields_ref, fields):
+    """Errors of a run against a reference run (both FP64 fields on the device)."""
+    E_ref, E = ref["relax"]["E"], rep["relax"]["E"]
+    S_ref, S = ref["saddle"]["E"], rep["saddle"]["E"]
+    out = {
+        "relax_E_rel": abs(E - E_ref) / abs(E_ref),
+        "relax_field_maxabs": float((fields["phi"] - fields_ref["phi"]).abs().max().item()),
+        "eig_abs": [abs(a - b) for a, b in zip(rep["eigenvalues"], ref["eigenvalues"])],
+        "saddle_E_rel": abs(S - S_ref) / abs(S_ref),
+        "saddle_E_abs": abs(S - S_ref),
+        "saddle_maxnorm_abs": abs(rep["saddle"]["maxabs"] - ref["saddle"]["maxabs"]),
+        "saddle_field_maxabs": float((fields["phi1"] - fields_ref["phi1"]).abs().max().item()),
+    }
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both")
+    ap.add_argument("--max-iter", type=int, default=100000)
+    ap.add_argument("--gn-tol", type=float, default=1e-8)
+    ap.add_argument("--polish", type=int, default=2000)
+    ap.add_argument("--calls", type=int, default=10)
+    ap.add_argument("--modes-polish", type=int, default=1)
+    ap.add_argument("--spm-maxiter", type=int, default=30000)
+    ap.add_argument("--spm-polish", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--quiet", action="store_true")
+    a = ap.parse_args(argv)
+    torch = spm._torch()
+    # icosahedral 30-vector star, |q| = 1 under k = n / L, amplitude 0.1, sphere of radius 6
+    phi0 = seeds.icosahedral_nucleus(a.N, 26.0, 1.0, 0.1, a.seed)
+    kw = dict(relax_kw=dict(gn_tol=a.gn_tol, max_iter=a.max_iter, polish=a.polish),
+              modes_kw=dict(calls=a.calls, modes_polish=a.modes_polish),
+              saddle_kw=dict(maxiter=a.spm_maxiter, spm_polish=a.spm_polish))
+    runs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    reps, flds = {}, {}
+    for p in runs:
+        _log(not a.quiet, "[%s]" % p)
+        reps[p], flds[p] = stage1(a.N, phi0, p, verbose=not a.quiet, **kw)
+        torch.cuda.empty_cache()
+    out = {"N": a.N, "runs": reps}
+    if len(runs) == 2:
+        out["mixed_vs_fp64"] = compare(reps["fp64"], reps["mixed"], flds["fp64"], flds["mixed"])
+        out["speedup"] = {s: reps["fp64"][k]["t"] / reps["mixed"][k]["t"]
+                          for s, k in (("relax", "relax"), ("modes", "modes"), ("saddle", "saddle"))}
+    txt = json.dumps(out, indent=1, default=float)
+    print(txt, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt + "\n")
+    return out
+
+
+if __name__ == "__main__":
+    main()

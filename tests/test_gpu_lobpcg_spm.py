@@ -1,7 +1,7 @@
 """GPU parity of LOBPCG (LOBPCG.cpp:10-487) and SPM (SPM.cpp:15-263) against
 the CPU FP64 oracle, plus closed-form and brute-force pins.
 
-Eigenvalues are compared in absolute terms at 1e-9 (north_star); eigenvectors
+Eigenvalues are compared in absolute terms at 1e-9; eigenvectors
 only through their span, since degenerate eigenspaces admit any basis."""
 import numpy as np
 import pytest

@@ -1,6 +1,6 @@
 """GPU parity of the HIP path (through the C ABI) against the CPU FP64 oracle.
 
-Tolerances (north_star): FP64 energy 1e-12 relative, gradient fields 1e-10 in
+Tolerances: FP64 energy 1e-12 relative, gradient fields 1e-10 in
 max-norm.  Transforms are held to 1e-13 relative (FP64) and 2e-6 (FP32).
 """
 import numpy as np
