@@ -67,8 +67,8 @@ def seed_field(seeds, N):
     return seeds.icosahedral_nucleus(N, L=26.0, kscale=1.0, amp=0.1, seed=0)
 
 
-def time_gpu(spm, torch, N, prec, steps, warmup, phi0):
-    g = spm.Grid(N, prec)
+def time_gpu(spm, torch, N, prec, steps, warmup, phi0, shape=0):
+    g = spm.Grid(N, prec, shape=shape)
     phi = torch.as_tensor(phi0).to(g.dtype).cuda()
     total = warmup + steps
     g.gf_begin(phi, total + 1, res_tol=-1.0, gn_tol=0.0)  # no early stop inside the timed region
@@ -109,6 +109,24 @@ def cpu_baseline(po, N, phi0, budget_s=20.0):
                 sample="%d FP64 gradient-flow steps of the oracle at %d^3 (%.1f s)" % (n, N, t))
 
 
+def serve_stream(n, rate):
+    """p50/p99 latency of a Poisson stream of relax / eigenmode / saddle requests
+    (64^3-256^3, mixed precision, continuous batching) and of the same stream
+    served one request at a time in FP64."""
+    serve = importlib.import_module("spm-iqc_amd.serve")
+    reqs = serve.make_requests(n, rate, seed=0)
+    states = serve.prepare_states(reqs)
+    recs, st = serve.serve(reqs, states, mixed=True)
+    base, sb = serve.serve_one_at_a_time(reqs, states)
+    keep = ("requests", "wall_s", "throughput_rps", "p50_s", "p99_s", "errors")
+    return {"requests": n, "rate_rps": rate, "mix": "60% relax (N 64/128/256, to |grad F| 1e-6 or 2000 steps), "
+            "25% 4-mode LOBPCG (N 64/128), 15% SPM 100 iterations (N 128)",
+            "p99_s": round(st["p99_s"], 4), "p50_s": round(st["p50_s"], 4),
+            "mixed_batched": {k: st[k] for k in keep},
+            "one_at_a_time_fp64": {k: sb[k] for k in keep},
+            "mixed_vs_fp64": serve.compare(recs, base)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +137,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 512 MiB device copy (calibrates HBM byte counters)")
+    ap.add_argument("--shape", type=int, default=-1,
+                    help="FFT pass shape (threads per pencil of the three passes); -1: measure them all "
+                         "first and use the fastest (the tuner)")
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="requests of the serving stream (p50/p99 latency); 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the workload is one 256^3 field); "
@@ -138,8 +162,16 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    r64 = time_gpu(spm, torch, N, spm.FP64, args.steps, args.warmup, phi0)
-    r32 = time_gpu(spm, torch, N, spm.FP32, args.steps, args.warmup, phi0)
+    shapes = spm.shapes(N)
+    tuned = {}
+    for prec in (spm.FP64, spm.FP32):
+        if args.shape >= 0:
+            tuned[prec] = (args.shape, None)
+        else:  # the tuner: every shape timed on 30 graph-replayed steps of a zero field
+            best, us = spm.tune(N, prec, nmem=1, steps=30)
+            tuned[prec] = (best, us)
+    r64 = time_gpu(spm, torch, N, spm.FP64, args.steps, args.warmup, phi0, tuned[spm.FP64][0])
+    r32 = time_gpu(spm, torch, N, spm.FP32, args.steps, args.warmup, phi0, tuned[spm.FP32][0])
 
     sps64 = args.steps / (r64["event_ms"] / 1e3)
     sps32 = args.steps / (r32["event_ms"] / 1e3)
@@ -147,6 +179,10 @@ def main():
     B32 = algorithmic_bytes_per_step(N, 4)
     ach64 = B64 * sps64
     ach32 = B32 * sps32
+
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_stream(args.serve_requests, args.serve_rate)
 
     cpu = None
     if not args.no_cpu:
@@ -169,17 +205,19 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded icosahedral nucleus, 30-vector star |q|=1, sphere r=6, tanh edge 1)",
         "config": {"workload": "semi-implicit gradient flow (get_min_u_F), one %d^3 field, L=26, k=n/L, "
-                               "Tau=-0.0061, Gamma=0.30, dt=0.3" % N, "N": N, "batch": 1},
+                               "Tau=-0.0061, Gamma=0.30, dt=0.3" % N, "N": N, "batch": 1,
+                   "fft_shape_fp64": shapes[tuned[spm.FP64][0]], "fft_shape_fp32": shapes[tuned[spm.FP32][0]]},
         "mixed": {"dtype": "f32 fields/FFTs, f64 sums", "value": round(sps32, 3),
                   "ms_per_step": round(r32["event_ms"] / args.steps, 4),
                   "speedup_vs_fp64": round(sps32 / sps64, 3),
                   "roofline_frac": round(ach32 / HBM_PEAK, 4)},
-        "roofline": {"bound": "hbm", "kernel": "gradient-flow step (5 launches: i-pass fwd+update+inv, "
-                                               "finalize, j-pass inv, k-pass c2r+f+r2c, j-pass fwd)",
+        "roofline": {"bound": "hbm", "kernel": "gradient-flow step (4 launches: i-axis fwd FFT + update + inv FFT; "
+                                               "j-axis inv FFT; k-axis c2r + f(phi) + r2c; j-axis fwd FFT)",
                      "achieved": round(ach64 / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(ach64 / HBM_PEAK, 4), "traffic": measured_traffic(N, "double"),
                      "algorithmic_bytes_per_step": B64},
         "cpu_baseline": cpu,
+        "serving": serving,
         "wall_s": {"fp64": round(r64["wall_s"], 4), "fp32": round(r32["wall_s"], 4)},
         "energy_last": {"fp64": float(r64["hist"][-1, 0]), "fp32": float(r32["hist"][-1, 0])},
     }

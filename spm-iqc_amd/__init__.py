@@ -105,6 +105,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise SpmError("libspm_iqc.so is not built: run `python spm-iqc_amd/build.py`")
+        # torch brings its own HIP runtime; it must be the one in the process
+        # before the library binds to libamdhip64, or the two runtimes clash
+        import torch  # noqa: F401
+
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
@@ -129,6 +133,7 @@ def _torch():
 
     if not torch.cuda.is_available():
         raise SpmError("no GPU: the spm-iqc_amd path runs only on the device")
+    torch.cuda.init()  # the HIP runtime is torch's: initialise it before the library's first call
     return torch
 
 

@@ -39,7 +39,8 @@ static int batch_capture(spm_batch *b)
 {
     Ctx &c = b->h->c;
     hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    // relaxed: other threads of the process may be allocating or launching meanwhile
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
     for (int s = 0; s < GRAPH_STEPS; ++s) c.g->gf_step(c);
     const hipError_t e = hipStreamEndCapture(c.stream, &g);
     if (e != hipSuccess) return spm_fail(SPM_EHIP, std::string("spm_batch_steps: capture: ") + hipGetErrorString(e));
@@ -88,6 +89,12 @@ int spm_batch_create(const spm_params *p, int capacity, int hist_cap, spm_batch 
         spm_ctx_free(b->h);
         delete b;
         return spm_fail(SPM_ENOMEM, "spm_batch_create: pinned control blocks");
+    }
+    // the step graph is captured once, here: every member starts stopped, and
+    // the graph reads members' state through their control blocks
+    if ((rc = batch_capture(b))) {
+        spm_batch_destroy(b);
+        return rc;
     }
     *out = b;
     return SPM_OK;
@@ -140,7 +147,6 @@ int spm_batch_steps(spm_batch *b, int nsteps)
     if (nsteps < 0) return spm_fail(SPM_EINVAL, "spm_batch_steps: nsteps >= 0");
     int s = 0;
     if (nsteps >= GRAPH_STEPS) {
-        if (!b->gexec) RCHK(batch_capture(b));
         for (; s + GRAPH_STEPS <= nsteps; s += GRAPH_STEPS) HIPCHK(hipGraphLaunch(b->gexec, c.stream));
     }
     for (; s < nsteps; ++s) c.g->gf_step(c);

@@ -233,6 +233,85 @@ __device__ __forceinline__ void stockham_stage(C2<T> (&v)[N / TPP], C2<T> *lds, 
     }
 }
 
+// The same stage with the exchange through a real-valued LDS area (half the
+// bytes, so twice the blocks fit on a CU): real parts first, then imaginary
+// parts, which wait in registers meanwhile.  No padding (column passes).
+template <typename T, int N, int TPP, int DIR, int R, int NS, bool LAST, int LS>
+__device__ __forceinline__ void stockham_stage_split(C2<T> (&v)[N / TPP], T *lds, const C2<T> *__restrict__ tw,
+                                                     int t)
+{
+    constexpr int E = N / TPP, NB = E / R;
+    static_assert(E % R == 0, "radix must divide the per-thread count");
+    if constexpr (!LAST) __syncthreads(); // earlier readers of this LDS area are done
+    T hold[LAST ? 1 : E];
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+        C2<T> a[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) a[q] = v[p + q * NB];
+        const int j = t + TPP * p;
+        const int jm = (NS > 1) ? j % NS : 0;
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                C2<T> w = tw[jm * q * (N / (NS * R))];
+                if constexpr (DIR > 0) w.y = -w.y;
+                a[q] = cmul(a[q], w);
+            }
+        }
+        dft<R, DIR>(a);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[p + q * NB] = a[q];
+        } else {
+            const int base = (j / NS) * NS * R + jm;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                lds[(base + q * NS) * LS] = a[q].x;
+                hold[p * R + q] = a[q].y;
+            }
+        }
+    }
+    if constexpr (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m].x = lds[(t + TPP * m) * LS];
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < NB; ++p) {
+            const int j = t + TPP * p;
+            const int jm = (NS > 1) ? j % NS : 0;
+            const int base = (j / NS) * NS * R + jm;
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[(base + q * NS) * LS] = hold[p * R + q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m].y = lds[(t + TPP * m) * LS];
+    }
+}
+
+template <typename T, int N, int TPP, int DIR, int LS>
+__device__ __forceinline__ void fft_pencil_split(C2<T> (&v)[N / TPP], T *lds, const C2<T> *__restrict__ tw, int t)
+{
+    using P = FftPlan<N, TPP>;
+    if constexpr (P::NST == 1) {
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, true, LS>(v, lds, tw, t);
+    } else if constexpr (P::NST == 2) {
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, true, LS>(v, lds, tw, t);
+    } else if constexpr (P::NST == 3) {
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, false, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R2, P::R0 * P::R1, true, LS>(v, lds, tw, t);
+    } else {
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, false, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R2, P::R0 * P::R1, false, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R3, P::R0 * P::R1 * P::R2, true, LS>(v, lds, tw, t);
+    }
+}
+
 // Full length-N transform of the pencil held in v (cyclic in, cyclic out).
 // Unnormalised in both directions.
 template <typename T, int N, int TPP, int DIR, int LS, bool PAD>

@@ -43,6 +43,10 @@ template <typename T, int N, int TPP_ = DefTPP<N>::value> struct Geo {
     static constexpr int B0 = 128 / (int)sizeof(C2<T>);
     static constexpr int B = (64 / TPP > B0) ? 64 / TPP : B0;
     static constexpr int COL_THREADS = B * TPP;
+    // FP64 column passes exchange real and imaginary parts separately through
+    // an LDS tile of N*B doubles: half the LDS per block, twice the blocks per CU
+    static constexpr bool XSPLIT = true;
+    using XT = typename std::conditional<XSPLIT, T, C2<T>>::type;
     // row passes: PP row pairs per block
     static constexpr int PP0 = (sizeof(T) == 8 ? 128 : 256) / TPP;
     static constexpr int PP = PP0 > 0 ? PP0 : 1;
@@ -304,7 +308,7 @@ template <typename T, int N> struct SOpGradFlowStep {
     {
         const int a = kfold(i, N), b = kfold(j, N);
         const int n2 = a * a + b * b + kz * kz;
-        const C2<T> gd = GD[n2];
+        const C2<T> gd = GD[n2]; // gathered here: fetching all E entries up front costs occupancy
         const double g = (double)gd.x, d = (double)gd.y, w = hweight<N>(kz);
         const double fr = (double)v.x * sc, fi = (double)v.y * sc;
         const double ur = (double)u.x, ui = (double)u.y;
@@ -591,7 +595,7 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
 {
     using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, B = G::B;
-    __shared__ C2<T> lds[N * B];
+    __shared__ typename G::XT lds[N * B];
     __shared__ C2<T> stw[N];
     __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
     const int mb = blockIdx.y; // batch member
@@ -623,7 +627,12 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
 #pragma unroll
         for (int m = 0; m < E; ++m) pre[m] = active ? ps[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
     }
-    if constexpr (FWD) fft_pencil<T, N, TPP, -1, B, false>(v, lds + c, stw, t);
+    if constexpr (FWD) {
+        if constexpr (G::XSPLIT)
+            fft_pencil_split<T, N, TPP, -1, B>(v, lds + c, stw, t);
+        else
+            fft_pencil<T, N, TPP, -1, B, false>(v, lds + c, stw, t);
+    }
     if (active) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -633,7 +642,12 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
             op(v[m], pre[SOp::PRELOAD ? m : 0], off + (long)e * pstride, i, j, kz, acc);
         }
     }
-    if constexpr (INV) fft_pencil<T, N, TPP, +1, B, false>(v, lds + c, stw, t);
+    if constexpr (INV) {
+        if constexpr (G::XSPLIT)
+            fft_pencil_split<T, N, TPP, +1, B>(v, lds + c, stw, t);
+        else
+            fft_pencil<T, N, TPP, +1, B, false>(v, lds + c, stw, t);
+    }
     if constexpr (STORE) {
         if (active)
 #pragma unroll

@@ -256,8 +256,8 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
         delete h;
         return code;
     };
-    if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess)
-        return bail(spm_fail(SPM_EHIP, "spm_create: hipStreamCreate failed (no GPU?)"));
+    if (hipError_t e = hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking); e != hipSuccess)
+        return bail(spm_fail(SPM_EHIP, std::string("spm_create: hipStreamCreate: ") + hipGetErrorString(e)));
     c.stream = h->own;
     // twiddles exp(-2 pi i e / N)
     {

@@ -102,7 +102,7 @@ int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
     if (!rc && (e = hipMemsetAsync(c.W, 0, (size_t)c.C * 2 * c.esz * nmem, c.stream)) != hipSuccess)
         fail("memset", e);
     for (int m = 0; !rc && m < nmem; ++m) spm_ctl_init(h, m, 2 * steps + 2, -1.0, 0.0); // never stops early
-    if (!rc && (e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+    if (!rc && (e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed)) != hipSuccess)
         fail("capture", e);
     if (!rc) {
         for (int s = 0; s < steps; ++s) c.g->gf_step(c);
@@ -169,7 +169,7 @@ int spm_tune(const spm_params *p, int nmem, int steps, int *best, double *us_per
     for (int v = 0; v < n; ++v) {
         double us = 0.0;
         const int rc = time_shape(p, nmem, steps, v, &us);
-        if (rc) return rc;
+        if (rc) return spm_fail(rc, "spm_tune: shape " + std::to_string(v) + ": " + spm_last_error());
         if (us_per_step) us_per_step[v] = us;
         if (bi < 0 || us < bt) {
             bi = v;

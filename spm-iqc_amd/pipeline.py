@@ -59,6 +59,13 @@ def _gradflow(g, phi, n, gn_tol, chunk, verbose, tag):
     return done, last
 
 
+def _device(torch, x, dtype):
+    """x (host array or tensor) as a contiguous device tensor of dtype (a copy)."""
+    if torch.is_tensor(x):
+        return x.to(device="cuda", dtype=dtype, copy=True).contiguous()
+    return torch.as_tensor(np.asarray(x)).to(device="cuda", dtype=dtype).contiguous()
+
+
 def relax(N, phi0, precision="fp64", gn_tol=1e-8, max_iter=100000, polish=2000, chunk=5000, params=None,
           verbose=False):
     """Gradient flow from phi0 until ||grad F|| <= gn_tol or max_iter steps
@@ -72,13 +79,13 @@ def relax(N, phi0, precision="fp64", gn_tol=1e-8, max_iter=100000, polish=2000, 
     n32 = 0
     if precision == "mixed":
         with spm.Grid(N, spm.FP32, **params) as g32:
-            p = torch.as_tensor(np.asarray(phi0)).to(torch.float32).cuda()
+            p = _device(torch, phi0, torch.float32)
             n32, last = _gradflow(g32, p, max(max_iter - polish, 0), gn_tol, chunk, verbose, "fp32")
             torch.cuda.current_stream().synchronize()
         phi = p.double()
         del p
     else:
-        phi = torch.as_tensor(np.asarray(phi0)).to(torch.float64).cuda()
+        phi = _device(torch, phi0, torch.float64)
     t1 = time.perf_counter()
     with spm.Grid(N, spm.FP64, **params) as g:
         n64, _ = _gradflow(g, phi, max_iter - n32, gn_tol, chunk, verbose, "fp64")
@@ -91,21 +98,24 @@ def relax(N, phi0, precision="fp64", gn_tol=1e-8, max_iter=100000, polish=2000, 
 
 
 def lowest_modes(N, phi, precision="fp64", k=8, nwant=4, calls=10, maxiter=21, seed=0, modes_polish=1,
-                 params=None, verbose=False):
+                 params=None, verbose=False, start=None):
     """hiosd_lobpcg_initialization from a seeded uniform [-1, 1) block of k
     fields, then `calls` hiosd_lobpcg_inip calls of maxiter iterations
     (main_SPM.cpp:81-91).  Mixed: the calls run in FP32 fields and the last
-    modes_polish of them in FP64.  Returns (ritz[:nwant], V FP64 (k, N, N, N), info)."""
+    modes_polish of them in FP64.  start: the (k, N, N, N) start block (host
+    array or device tensor) instead of the seeded one.
+    Returns (ritz[:nwant], V FP64 (k, N, N, N), info)."""
     torch = spm._torch()
     params = dict(DEFAULTS, **(params or {}))
-    start = np.random.default_rng(seed).uniform(-1.0, 1.0, (k, N, N, N))
+    if start is None:
+        start = np.random.default_rng(seed).uniform(-1.0, 1.0, (k, N, N, N))
     info = {"k": k, "calls": calls, "maxiter": maxiter}
     t0 = time.perf_counter()
     n32 = calls - modes_polish if precision == "mixed" else 0
     ritz = None
     if n32 > 0:
         with spm.Grid(N, spm.FP32, **params) as g32:
-            V = g32.lobpcg_init(torch.as_tensor(start).to(torch.float32).cuda())
+            V = g32.lobpcg_init(torch.as_tensor(start).to(device="cuda", dtype=torch.float32).contiguous())
             del start
             p32 = phi.float()
             for c in range(n32):
@@ -115,7 +125,7 @@ def lowest_modes(N, phi, precision="fp64", k=8, nwant=4, calls=10, maxiter=21, s
             del p32
     with spm.Grid(N, spm.FP64, **params) as g:
         if n32 <= 0:
-            V = g.lobpcg_init(torch.as_tensor(start).to(torch.float64).cuda())
+            V = g.lobpcg_init(torch.as_tensor(start).to(device="cuda", dtype=torch.float64).contiguous())
             del start
         for c in range(calls - max(n32, 0)):
             ritz = g.lobpcg(phi, V, maxiter)

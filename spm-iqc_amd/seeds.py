@@ -101,3 +101,23 @@ def crystal(N: int, kind: str, L: float = 26.0, kscale: float = 1.0, amp: float 
     """Periodic seed from one of the stars in STARS (amplitude amp, seeded phases)."""
     modes = grid_modes(STARS[kind](), L, kscale)
     return plane_waves(N, modes, amp, seed)
+
+
+def device_field(torch, N: int, kind: str, L: float = 26.0, kscale: float = 1.0, amp: float = 0.1, seed: int = 0,
+                 dtype=None, radius: float = 6.0, width: float = 1.0):
+    """The same seeds evaluated on the GPU in FP64 (phases drawn on the host
+    exactly as plane_waves draws them): kind is a key of STARS (periodic
+    crystal) or "nucleus" (the icosahedral star inside the tanh sphere).
+    Used where seeding must not hold the host (serving, batched sweeps)."""
+    star = icosahedral_star() if kind == "nucleus" else STARS[kind]()
+    modes = grid_modes(star, L, kscale)
+    theta = np.random.default_rng(seed).uniform(0.0, 2.0 * np.pi, size=len(modes))
+    i = torch.arange(N, dtype=torch.float64, device="cuda") * (2.0 * np.pi / N)
+    out = torch.zeros((N, N, N), dtype=torch.float64, device="cuda")
+    for (a, b, c), th in zip(modes, theta):
+        out += amp * torch.cos((a * i)[:, None, None] + (b * i)[None, :, None] + (c * i)[None, None, :] + float(th))
+    if kind == "nucleus":
+        r = torch.arange(N, dtype=torch.float64, device="cuda") * (L / N) - L / 2.0
+        d = torch.sqrt(r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2)
+        out *= 0.5 * (1.0 - torch.tanh((d - radius) / width))
+    return out if dtype is None else out.to(dtype)

@@ -71,9 +71,9 @@ def make_requests(n=500, rate=8.0, seed=0, relax_sizes=(64, 128, 256), modes_siz
     return reqs
 
 
-def relax_seed(r):
-    job = dict(N=r["N"], L=L, kind=r["seed_kind"], seed=r["seed"])
-    return sweep.seed_field(job)
+def relax_seed(r, dtype=None):
+    """The request's seed field, generated on the device."""
+    return sweep.seed_field(dict(N=r["N"], L=L, kind=r["seed_kind"], seed=r["seed"]), dtype)
 
 
 def prepare_states(reqs, verbose=False):
@@ -84,7 +84,7 @@ def prepare_states(reqs, verbose=False):
     sizes = sorted({r["N"] for r in reqs if r["kind"] in ("modes", "saddle")})
     states = {}
     for N in sizes:
-        phi0 = seeds.icosahedral_nucleus(N, L, KSCALE, 0.1, 0)
+        phi0 = seeds.device_field(torch, N, "nucleus", L, KSCALE, 0.1, 0)
         phi, info = pipeline.relax(N, phi0, "fp64", gn_tol=1e-8, max_iter=4000, params=PARAMS)
         ritz, V, _ = pipeline.lowest_modes(N, phi, "fp64", k=8, nwant=4, calls=3, params=PARAMS)
         states[N] = dict(phi=phi, v1=V[0].clone())
@@ -110,9 +110,15 @@ def footprint(r, prec_bytes=4):
 # worker requests (one context per request, on the worker's stream)
 # ---------------------------------------------------------------------------
 def run_modes(r, states, mixed):
-    phi = states[r["N"]]["phi"]
-    ritz, V, info = pipeline.lowest_modes(r["N"], phi, "mixed" if mixed else "fp64", k=8, nwant=4,
-                                          calls=r["calls"], seed=r["seed"], modes_polish=1, params=PARAMS)
+    torch = spm._torch()
+    N, k = r["N"], 8
+    phi = states[N]["phi"]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(r["seed"])
+    start = torch.rand((k, N, N, N), generator=gen, dtype=torch.float64, device="cuda") * 2.0 - 1.0
+    ritz, V, info = pipeline.lowest_modes(N, phi, "mixed" if mixed else "fp64", k=k, nwant=4,
+                                          calls=r["calls"], seed=r["seed"], modes_polish=1, params=PARAMS,
+                                          start=start)
     del V
     return dict(eigenvalues=[float(x) for x in ritz])
 
@@ -128,7 +134,7 @@ def run_saddle(r, states, mixed):
 def run_relax_single(r):
     torch = spm._torch()
     with spm.Grid(r["N"], spm.FP64, L=L, kscale=KSCALE, Tau=r["Tau"], Gamma=r["Gamma"]) as g:
-        phi = torch.as_tensor(relax_seed(r)).cuda()
+        phi = relax_seed(r)
         it, hist = g.gradflow(phi, r["max_iter"], -1.0, r["tol"], check_every=128)
     return dict(iters=it, E=float(hist[-1, 0]), gn=float(hist[-1, 1]))
 
@@ -192,6 +198,7 @@ def serve(reqs, states, mixed=True, capacity=None, chunk=32, workers=4, budget_b
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     nxt = 0
+    shown = 0
     while len(recs) < len(reqs):
         now = time.perf_counter() - t0
         # arrivals
@@ -199,7 +206,7 @@ def serve(reqs, states, mixed=True, capacity=None, chunk=32, workers=4, budget_b
             r = pending[nxt]
             nxt += 1
             if r["kind"] == "relax":
-                phi = torch.as_tensor(relax_seed(r)).to(torch.float32 if mixed else torch.float64).cuda()
+                phi = relax_seed(r, torch.float32 if mixed else torch.float64)
                 stages[(r["N"], first)].queue.append((r, phi, r["max_iter"], r["tol"]))
             else:
                 waiting.append(r)
@@ -255,8 +262,12 @@ def serve(reqs, states, mixed=True, capacity=None, chunk=32, workers=4, budget_b
             # nothing on the batches: wait for the next arrival or a worker
             wait = pending[nxt]["t"] - (time.perf_counter() - t0) if nxt < len(pending) else 0.005
             time.sleep(min(max(wait, 0.0), 0.005))
-        if verbose and len(recs) and len(recs) % 50 == 0:
-            pass
+        if verbose and time.perf_counter() - t0 >= shown:
+            shown += 2.0
+            occ = " ".join("%d%s:%d/%d+%d" % (N, "s" if p == spm.FP32 else "d", len(s.slot_job), s.b.capacity,
+                                                len(s.queue)) for (N, p), s in stages.items())
+            print("  t %5.1f s: arrived %d, served %d, workers %d running %d waiting; batches %s" % (
+                time.perf_counter() - t0, nxt, len(recs), len(in_flight), len(waiting), occ), flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     pool.close()
@@ -287,6 +298,8 @@ def serve_one_at_a_time(reqs, states, verbose=False):
         rec = dict(id=r["id"], kind=r["kind"], N=r["N"], t=r["t"], t_done=t_done, latency=t_done - r["t"])
         rec.update(out)
         recs.append(rec)
+        if verbose and len(recs) % 50 == 0:
+            print("  one at a time: %d / %d at t = %.1f s" % (len(recs), len(reqs), t_done), flush=True)
     wall = time.perf_counter() - t0
     recs.sort(key=lambda x: x["id"])
     return recs, summarize(recs, wall)
@@ -304,17 +317,42 @@ def summarize(recs, wall):
                 mean_s=float(lat.mean()), errors=sum(1 for r in recs if "error" in r), per_kind=per)
 
 
-def compare(a, b):
-    """Largest differences between two served copies of the same stream."""
-    out = {"relax_E_rel": 0.0, "eig_abs": 0.0, "saddle_E_rel": 0.0}
+def compare(a, b, tol=None):
+    """Differences between two served copies of the same stream (a against b).
+    Relaxations: where both reached their tolerance, energies compared
+    absolutely at the disordered minimum (|E| <= 1e-10) and relatively at
+    ordered ones; where a run stopped at its step budget instead, the
+    relative difference of the energy reached after the same budget."""
+    conv_abs, conv_rel, capped_rel, n_conv, n_cap = [], [], [], 0, 0
+    eig, sad = [], []
     for x, y in zip(a, b):
         if x["kind"] == "relax":
-            out["relax_E_rel"] = max(out["relax_E_rel"], abs(x["E"] - y["E"]) / max(abs(y["E"]), 1e-300))
+            t = tol if tol is not None else 1e-6
+            if x["gn"] <= t and y["gn"] <= t:
+                n_conv += 1
+                if abs(y["E"]) <= 1e-10:
+                    conv_abs.append(abs(x["E"] - y["E"]))
+                else:
+                    conv_rel.append(abs(x["E"] - y["E"]) / abs(y["E"]))
+            else:
+                n_cap += 1
+                capped_rel.append(abs(x["E"] - y["E"]) / max(abs(y["E"]), 1e-300))
         elif x["kind"] == "modes":
-            out["eig_abs"] = max(out["eig_abs"], float(np.max(np.abs(np.subtract(x["eigenvalues"], y["eigenvalues"])))))
+            eig.append(float(np.max(np.abs(np.subtract(x["eigenvalues"], y["eigenvalues"])))))
         elif "E" in x and "E" in y:
-            out["saddle_E_rel"] = max(out["saddle_E_rel"], abs(x["E"] - y["E"]) / max(abs(y["E"]), 1e-300))
-    return out
+            sad.append(abs(x["E"] - y["E"]) / max(abs(y["E"]), 1e-300))
+
+    def mx(v):
+        return float(max(v)) if v else None
+
+    def med(v):
+        return float(np.median(v)) if v else None
+
+    return {"relax_converged": n_conv, "relax_converged_E_abs_max_disordered": mx(conv_abs),
+            "relax_converged_E_rel_max_ordered": mx(conv_rel), "relax_budget_capped": n_cap,
+            "relax_capped_E_rel_median": med(capped_rel), "relax_capped_E_rel_max": mx(capped_rel),
+            "eig_abs_max": mx(eig), "eig_abs_median": med(eig), "saddle_E_rel_max": mx(sad),
+            "saddle_E_rel_median": med(sad)}
 
 
 def main(argv=None):
@@ -341,7 +379,7 @@ def main(argv=None):
     print(json.dumps({k: v for k, v in out.items()}), flush=True)
     if a.out:
         with open(a.out, "w") as f:
-            json.dump(dict(out, records=recs), f, indent=1)
+            json.dump(dict(out, records=recs, records_fp64=base if a.baseline else None), f, indent=1)
     return out
 
 

@@ -176,3 +176,12 @@ def test_sweep_mixed_converges_to_fp64_minimum(sweep):
     for r, q in zip(res, ref):
         assert r["iters_fp32"] > 0 and r["gn"] <= 1e-8 and q["gn"] <= 1e-8, (r, q)
         assert abs(r["E"] - q["E"]) <= 1e-12, (r, q)
+
+
+@pytest.mark.parametrize("kind", ["bcc", "fcc", "lam", "nucleus"])
+def test_device_seeds_match_host_seeds(seeds, torch, kind):
+    N, ks = 32, 2 * np.pi
+    dev = seeds.device_field(torch, N, kind, 26.0, ks, 0.1, 5).cpu().numpy()
+    host = seeds.icosahedral_nucleus(N, 26.0, ks, 0.1, 5) if kind == "nucleus" else \
+        seeds.crystal(N, kind, 26.0, ks, 0.1, 5)
+    assert np.abs(dev - host).max() <= 1e-12
