@@ -11,7 +11,7 @@ namespace spm {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int GRAM_BLOCKS = 512;  // blocks of 4 waves; fixed, so the summation order is fixed
+constexpr int GRAM_BLOCKS = 1024; // blocks of 4 waves; fixed, so the summation order is fixed
 constexpr int GRAM_TILE = 32;     // the Gram is computed as a padded 32 x 32 block
 
 // Each wave accumulates C = X^T Y over its 16-point chunks with
@@ -37,9 +37,9 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb) yb[tb] = (tb * 16 + row < mb) ? (const T *)Y.p[tb * 16 + row] : nullptr;
     const long nchunk = R / 16;
-    for (long q = (long)blockIdx.x * 4 + wid; q < nchunk; q += (long)gridDim.x * 4) {
+    // lane data of chunk q: 4 consecutive points r0 + 4 (l / 16) .. +3 of its row's field
+    auto load = [&](long q, double (&a)[TA][4], double (&b)[TB][4]) {
         const long r0 = q * 16 + 4 * kq;
-        double a[TA][4], b[TB][4];
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta) {
             if (xa[ta]) {
@@ -68,6 +68,15 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
                 b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = 0.0;
             }
         }
+    };
+    // one chunk ahead: the next chunk's loads are in flight during this chunk's MFMAs
+    const long step = (long)gridDim.x * 4;
+    long q = (long)blockIdx.x * 4 + wid;
+    double a[TA][4], b[TB][4];
+    if (q < nchunk) load(q, a, b);
+    for (; q < nchunk; q += step) {
+        double an[TA][4], bn[TB][4];
+        if (q + step < nchunk) load(q + step, an, bn);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -75,6 +84,14 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
 #pragma unroll
                 for (int tb = 0; tb < TB; ++tb)
                     acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], b[tb][s], acc[ta][tb], 0, 0, 0);
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[ta][s] = an[ta][s];
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) b[tb][s] = bn[tb][s];
     }
 #pragma unroll
     for (int ta = 0; ta < TA; ++ta)
@@ -98,38 +115,71 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
     }
 }
 
-// out[e] = scale * sum over blocks of part[b][e], in block order
+// out[e] = scale * sum over blocks of part[b][e]: 64 entries per block, the
+// blocks b = g (mod 4) summed by thread group g in a fixed order, then the 4
+// group sums combined in a fixed order (the result does not depend on timing).
 __global__ void __launch_bounds__(256) k_gram_reduce(const double *__restrict__ part, int nblk, double scale,
                                                     double *__restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= GRAM_TILE * GRAM_TILE) return;
+    __shared__ double sh[4][64];
+    const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + el;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(long)b * GRAM_TILE * GRAM_TILE + e];
-    out[e] = s * scale;
+#pragma unroll 8
+    for (int b = g; b < nblk; b += 4) s += part[(long)b * GRAM_TILE * GRAM_TILE + e];
+    sh[g][el] = s;
+    __syncthreads();
+    if (g == 0) out[e] = ((sh[0][el] + sh[1][el]) + (sh[2][el] + sh[3][el])) * scale;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_combine(FieldPtrs in, int nin, FieldPtrsW out, int nout,
+// Operands of k_combine, uploaded to the device with its coefficients: field
+// pointers and coefficients are then read with uniform (scalar) loads.
+struct CombineArgs {
+    const void *in[BLK_MAX];
+    void *out[BLK_MAX];
+};
+
+// out_{i0+o} = sum_a coef[a*nout + i0+o] in_a + shift[i0+o], o < NO: four
+// consecutive points per thread, all inputs read once per point (so an
+// output may alias an input).
+template <typename T, int NO>
+__global__ void __launch_bounds__(256) k_combine(const CombineArgs *__restrict__ args, int nin, int nout, int i0,
                                                 const double *__restrict__ coef, const double *__restrict__ shift,
-                                                long R)
+                                                long R4)
 {
-    __shared__ double cs[BLK_MAX * BLK_MAX];
-    __shared__ double ss[BLK_MAX];
-    for (int e = threadIdx.x; e < nin * nout; e += blockDim.x) cs[e] = coef[e];
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) ss[e] = shift ? shift[e] : 0.0;
-    __syncthreads();
-    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (long)gridDim.x * blockDim.x) {
-        double x[BLK_MAX];
+    for (long r4 = (long)blockIdx.x * blockDim.x + threadIdx.x; r4 < R4; r4 += (long)gridDim.x * blockDim.x) {
+        double acc[NO][4];
 #pragma unroll
-        for (int a = 0; a < BLK_MAX; ++a) x[a] = a < nin ? (double)((const T *)in.p[a])[r] : 0.0;
-#pragma unroll 4
-        for (int i = 0; i < nout; ++i) {
-            double s = ss[i];
+        for (int o = 0; o < NO; ++o) {
+            const double sh = shift ? shift[i0 + o] : 0.0;
 #pragma unroll
-            for (int a = 0; a < BLK_MAX; ++a)
-                if (a < nin) s += cs[a * nout + i] * x[a];
-            ((T *)out.p[i])[r] = (T)s;
+            for (int k = 0; k < 4; ++k) acc[o][k] = sh;
+        }
+        for (int a = 0; a < nin; ++a) {
+            double x[4];
+            if constexpr (sizeof(T) == 4) {
+                const float4 u = ((const float4 *)args->in[a])[r4];
+                x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w;
+            } else {
+                const double2 u = ((const double2 *)args->in[a])[2 * r4], w = ((const double2 *)args->in[a])[2 * r4 + 1];
+                x[0] = u.x; x[1] = u.y; x[2] = w.x; x[3] = w.y;
+            }
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                const double cf = coef[a * nout + i0 + o];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[o][k] += cf * x[k];
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            if constexpr (sizeof(T) == 4) {
+                ((float4 *)args->out[i0 + o])[r4] = float4{(float)acc[o][0], (float)acc[o][1], (float)acc[o][2],
+                                                           (float)acc[o][3]};
+            } else {
+                ((double2 *)args->out[i0 + o])[2 * r4] = double2{acc[o][0], acc[o][1]};
+                ((double2 *)args->out[i0 + o])[2 * r4 + 1] = double2{acc[o][2], acc[o][3]};
+            }
         }
     }
 }
@@ -204,7 +254,8 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     }
 #undef GRAM_LAUNCH
     double *dout = small_dev(h);
-    hipLaunchKernelGGL(k_gram_reduce, dim3(4), dim3(256), 0, c.stream, part, nblk, 1.0 / (double)c.R, dout);
+    hipLaunchKernelGGL(k_gram_reduce, dim3(GRAM_TILE * GRAM_TILE / 64), dim3(256), 0, c.stream, part, nblk,
+                       1.0 / (double)c.R, dout);
     RCHK(spm_launch_status("block_gram"));
     std::vector<double> full(GRAM_TILE * GRAM_TILE);
     HIPCHK(hipMemcpyAsync(full.data(), dout, sizeof(double) * full.size(), hipMemcpyDeviceToHost, c.stream));
@@ -219,25 +270,52 @@ int block_combine(spm_ctx *h, const void *const *in, int nin, void *const *out, 
 {
     Ctx &c = h->c;
     if (nin < 1 || nout < 1 || nin > BLK_MAX || nout > BLK_MAX) return spm_fail(SPM_EINVAL, "block_combine");
+    constexpr int NO = 8; // outputs per launch
+    if (nout > NO)        // later launches would read inputs that earlier ones overwrote
+        for (int a = 0; a < nin; ++a)
+            for (int i = 0; i < nout; ++i)
+                if (in[a] == out[i]) return spm_fail(SPM_EINVAL, "block_combine: in place with more than 8 outputs");
     RCHK(ensure_bscr(h));
-    // coefficients: uploaded after the stream has drained (the scratch may be in use by queued work)
-    double *dc = small_dev(h) + GRAM_TILE * GRAM_TILE;
-    std::vector<double> buf(nin * nout + nout, 0.0);
-    std::memcpy(buf.data(), coef, sizeof(double) * nin * nout);
-    if (shift) std::memcpy(buf.data() + nin * nout, shift, sizeof(double) * nout);
+    // operands and coefficients: uploaded after the stream has drained (the
+    // scratch may be in use by queued work)
+    double *dsm = small_dev(h) + GRAM_TILE * GRAM_TILE;
+    CombineArgs ca{};
+    for (int a = 0; a < nin; ++a) ca.in[a] = in[a];
+    for (int i = 0; i < nout; ++i) ca.out[i] = out[i];
+    const size_t nargs = (sizeof(CombineArgs) + sizeof(double) - 1) / sizeof(double);
+    std::vector<double> buf(nargs + nin * nout + nout, 0.0);
+    std::memcpy(buf.data(), &ca, sizeof(ca));
+    std::memcpy(buf.data() + nargs, coef, sizeof(double) * nin * nout);
+    if (shift) std::memcpy(buf.data() + nargs + nin * nout, shift, sizeof(double) * nout);
     HIPCHK(hipStreamSynchronize(c.stream));
-    HIPCHK(hipMemcpy(dc, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
-    FieldPtrs ip{};
-    FieldPtrsW op{};
-    for (int a = 0; a < nin; ++a) ip.p[a] = in[a];
-    for (int i = 0; i < nout; ++i) op.p[i] = out[i];
-    const int nb = (int)std::min<long>(2048, (c.R + 255) / 256);
-    if (c.prec == SPM_FP64)
-        hipLaunchKernelGGL(k_combine<double>, dim3(nb), dim3(256), 0, c.stream, ip, nin, op, nout, dc,
-                           shift ? dc + nin * nout : nullptr, c.R);
-    else
-        hipLaunchKernelGGL(k_combine<float>, dim3(nb), dim3(256), 0, c.stream, ip, nin, op, nout, dc,
-                           shift ? dc + nin * nout : nullptr, c.R);
+    HIPCHK(hipMemcpy(dsm, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+    const CombineArgs *dargs = (const CombineArgs *)dsm;
+    const double *dc = dsm + nargs, *ds = shift ? dsm + nargs + nin * nout : nullptr;
+    const long R4 = c.R / 4; // N^3, N >= 4: a multiple of 4
+    const int nb = (int)std::min<long>(2048, (R4 + 255) / 256);
+    for (int i0 = 0; i0 < nout; i0 += NO) {
+        const int no = std::min(NO, nout - i0);
+#define COMBINE_LAUNCH(T, K)                                                                                   \
+    hipLaunchKernelGGL((k_combine<T, K>), dim3(nb), dim3(256), 0, c.stream, dargs, nin, nout, i0, dc, ds, R4)
+#define COMBINE_BY_NO(T)                                                                                       \
+    switch (no) {                                                                                              \
+    case 1: COMBINE_LAUNCH(T, 1); break;                                                                       \
+    case 2: COMBINE_LAUNCH(T, 2); break;                                                                       \
+    case 3: COMBINE_LAUNCH(T, 3); break;                                                                       \
+    case 4: COMBINE_LAUNCH(T, 4); break;                                                                       \
+    case 5: COMBINE_LAUNCH(T, 5); break;                                                                       \
+    case 6: COMBINE_LAUNCH(T, 6); break;                                                                       \
+    case 7: COMBINE_LAUNCH(T, 7); break;                                                                       \
+    default: COMBINE_LAUNCH(T, 8); break;                                                                      \
+    }
+        if (c.prec == SPM_FP64) {
+            COMBINE_BY_NO(double)
+        } else {
+            COMBINE_BY_NO(float)
+        }
+#undef COMBINE_BY_NO
+#undef COMBINE_LAUNCH
+    }
     return spm_launch_status("block_combine");
 }
 
