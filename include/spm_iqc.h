@@ -107,10 +107,13 @@ int spm_gradflow(spm_ctx *c, void *phi, int max_iter, double res_tol, double gn_
                  double *hist, int *iters);
 
 /* The same loop in three asynchronous parts, for callers that drive the
- * iterations themselves (graph capture, timing): begin = set-up of
- * get_min_u_F's state from phi; steps = enqueue nsteps iterations (those
- * past the stop condition do nothing); end = write the final field to phi
- * (if not NULL), synchronise, return the history as spm_gradflow does. */
+ * iterations themselves (timing, overlap): begin = set-up of get_min_u_F's
+ * state from phi; steps = enqueue nsteps iterations (those past the stop
+ * condition do nothing); end = write the final field to phi (if not NULL),
+ * synchronise, return the history as spm_gradflow does.
+ * Iterations are replayed 16 at a time from a HIP graph on the context's own
+ * stream, ordered after earlier work and before later work of the context's
+ * stream by events (spm_gradflow does the same). */
 int spm_gf_begin(spm_ctx *c, const void *phi, int max_iter, double res_tol, double gn_tol);
 int spm_gf_steps(spm_ctx *c, int nsteps);
 int spm_gf_end(spm_ctx *c, void *phi, double *hist, int *iters);

@@ -19,7 +19,18 @@ struct spm_ctx {
     // block-operation scratch (Gram partials, coefficients)
     double *bscr = nullptr;
     size_t bscr_bytes = 0;
+    // gradient-flow steps replayed from a graph captured on `own` (valid while
+    // the history buffer it writes stays where it was at capture)
+    hipGraphExec_t gf_exec = nullptr;
+    const void *gf_exec_hist = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr; // ordering with the caller's stream
 };
+
+// Enqueue n gradient-flow iterations: whole graphs of GF_GRAPH_STEPS on the
+// context's own stream (ordered after and before the caller's stream by
+// events), the remainder as single launches.
+constexpr int GF_GRAPH_STEPS = 16;
+int spm_gf_enqueue(spm_ctx *h, long n);
 
 int spm_fail(int code, const std::string &msg);
 // context of nmem batch members (1: a single field) with hist_cap iterations of history each

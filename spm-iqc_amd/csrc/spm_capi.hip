@@ -155,6 +155,9 @@ static void free_all(spm_ctx *h)
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->h_pin) (void)hipHostFree(h->h_pin);
+    if (h->gf_exec) (void)hipGraphExecDestroy(h->gf_exec);
+    if (h->ev_in) (void)hipEventDestroy(h->ev_in);
+    if (h->ev_out) (void)hipEventDestroy(h->ev_out);
     for (void *p : {h->ws, (void *)h->bscr})
         if (p) (void)hipFree(p);
     if (h->own) (void)hipStreamDestroy(h->own);
@@ -434,6 +437,54 @@ int spm_c2r(spm_ctx *h, const void *S, void *x)
 
 } // extern "C"
 
+int spm_gf_enqueue(spm_ctx *h, long n)
+{
+    Ctx &c = h->c;
+    if (n < GF_GRAPH_STEPS) {
+        for (long s = 0; s < n; ++s) c.g->gf_step(c);
+        return spm_launch_status("gradient-flow steps");
+    }
+    hipStream_t caller = c.stream;
+    if (!h->gf_exec || h->gf_exec_hist != c.hist) {
+        if (h->gf_exec) {
+            HIPCHK(hipGraphExecDestroy(h->gf_exec));
+            h->gf_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        c.stream = h->own;
+        hipError_t e = hipStreamBeginCapture(h->own, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            for (int s = 0; s < GF_GRAPH_STEPS; ++s) c.g->gf_step(c);
+            e = hipStreamEndCapture(h->own, &g);
+        }
+        c.stream = caller;
+        if (e == hipSuccess) e = hipGraphInstantiate(&h->gf_exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            h->gf_exec = nullptr;
+            return spm_fail(SPM_EHIP, std::string("gradient-flow graph: ") + hipGetErrorString(e));
+        }
+        h->gf_exec_hist = c.hist;
+    }
+    if (!h->ev_in) HIPCHK(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+    if (!h->ev_out) HIPCHK(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
+    if (caller != h->own) {
+        HIPCHK(hipEventRecord(h->ev_in, caller));
+        HIPCHK(hipStreamWaitEvent(h->own, h->ev_in, 0));
+    }
+    long s = 0;
+    for (; s + GF_GRAPH_STEPS <= n; s += GF_GRAPH_STEPS) HIPCHK(hipGraphLaunch(h->gf_exec, h->own));
+    c.stream = h->own;
+    for (; s < n; ++s) c.g->gf_step(c);
+    c.stream = caller;
+    RCHK(spm_launch_status("gradient-flow steps"));
+    if (caller != h->own) {
+        HIPCHK(hipEventRecord(h->ev_out, h->own));
+        HIPCHK(hipStreamWaitEvent(caller, h->ev_out, 0));
+    }
+    return SPM_OK;
+}
+
 int spm_read_scalars(spm_ctx *h, const double *dev, int n, double *out)
 {
     Ctx &c = h->c;
@@ -503,9 +554,8 @@ int spm_gradflow(spm_ctx *h, void *phi, int max_iter, double res_tol, double gn_
         long queued = 0;
         while (queued < max_iter) {
             const long n = std::min<long>(check_every, max_iter - queued);
-            for (long s = 0; s < n; ++s) c.g->gf_step(c);
+            if ((rc = spm_gf_enqueue(h, n))) return rc;
             queued += n;
-            if ((rc = spm_launch_status("spm_gradflow"))) return rc;
             HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
             HIPCHK(hipStreamSynchronize(c.stream));
             if (((Ctl *)h->h_pin)->stop) break;
@@ -539,8 +589,7 @@ int spm_gf_steps(spm_ctx *h, int nsteps)
 {
     CHECK_CTX(h);
     if (nsteps < 0) return spm_fail(SPM_EINVAL, "spm_gf_steps: nsteps >= 0");
-    for (int s = 0; s < nsteps; ++s) c.g->gf_step(c);
-    return spm_launch_status("spm_gf_steps");
+    return spm_gf_enqueue(h, nsteps);
 }
 
 int spm_gf_end(spm_ctx *h, void *phi, double *hist, int *iters)

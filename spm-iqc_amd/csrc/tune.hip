@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -81,8 +82,24 @@ void record(const std::string &ar, int N, int prec, int nmem, int shape, double 
     }
 }
 
-// Time `steps` iterations of a batch of nmem zero fields with shape v,
-// replayed from one captured graph (after one untimed replay).
+// U <- pseudo-random spectrum of amplitude amp (a hash of the index), so the
+// timed steps move realistic data (all-zero fields run measurably faster).
+template <typename T> __global__ void k_fill_noise(C2<T> *p, long n, double amp)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double a = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        const double b = (double)((z << 21) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        p[i] = C2<T>{(T)(amp * a), (T)(amp * b)};
+    }
+}
+
+// Time `steps` iterations of a batch of nmem fields with shape v, replayed
+// from one captured graph (after one untimed replay), starting from a
+// noise spectrum whose field has r.m.s. about 0.1.
 int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
 {
     spm_params q = *p;
@@ -98,9 +115,17 @@ int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
         rc = spm_fail(SPM_EHIP, std::string("spm_tune: ") + what + ": " + hipGetErrorString(e));
     };
     hipError_t e;
-    if ((e = hipMemsetAsync(c.U, 0, (size_t)c.C * 2 * c.esz * nmem, c.stream)) != hipSuccess) fail("memset", e);
-    if (!rc && (e = hipMemsetAsync(c.W, 0, (size_t)c.C * 2 * c.esz * nmem, c.stream)) != hipSuccess)
-        fail("memset", e);
+    if ((e = hipMemsetAsync(c.W, 0, (size_t)c.C * 2 * c.esz * nmem, c.stream)) != hipSuccess) fail("memset", e);
+    if (!rc) {
+        const double amp = 0.2 / std::sqrt(2.0 * (double)c.R); // field r.m.s. ~ 0.1
+        if (c.prec == SPM_FP64)
+            hipLaunchKernelGGL(k_fill_noise<double>, dim3(2048), dim3(256), 0, c.stream, (C2<double> *)c.U,
+                               c.C * nmem, amp);
+        else
+            hipLaunchKernelGGL(k_fill_noise<float>, dim3(2048), dim3(256), 0, c.stream, (C2<float> *)c.U,
+                               c.C * nmem, amp);
+        if ((e = hipGetLastError()) != hipSuccess) fail("noise", e);
+    }
     for (int m = 0; !rc && m < nmem; ++m) spm_ctl_init(h, m, 2 * steps + 2, -1.0, 0.0); // never stops early
     if (!rc && (e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed)) != hipSuccess)
         fail("capture", e);

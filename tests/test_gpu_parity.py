@@ -179,3 +179,22 @@ def test_gradflow_fp32(spm, po, seeds, torch, N):
         itg, hist = g.gradflow(pt, steps, 1e-16, 0.0, check_every=10)
     assert itg == steps
     assert np.all(np.abs(hist[:, 0] - E) <= 1e-5 * np.abs(E))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gradflow_graph_replay_equals_single_launches(spm, seeds, prec):
+    """Iterations replayed from the captured graph (16 at a time, host checks
+    and copies between replays) give exactly the single-launch iterations."""
+    import torch
+
+    N, n = 32, 53
+    phi = seeds.crystal(N, "bcc", 26.0, 2 * np.pi, 0.1, 4)
+    out = []
+    for check in (1, 32):  # 1: every iteration launched on its own; 32: two graph replays per check
+        with spm.Grid(N, prec, kscale=2 * np.pi, Tau=0.005) as g:
+            t = torch.as_tensor(phi).to(g.dtype).cuda()
+            it, hist = g.gradflow(t, n, -1.0, 0.0, check_every=check)
+            out.append((it, hist.copy(), t.cpu().numpy()))
+    (i1, h1, f1), (i2, h2, f2) = out
+    assert i1 == i2 == n
+    assert np.array_equal(h1, h2) and np.array_equal(f1, f2)
