@@ -115,10 +115,16 @@ template <typename T> struct OpGradFlowRow {
     }
     __device__ __forceinline__ T operator()(T x, long g, double *acc) const
     {
-        const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
-        acc[0] += -0.5 * Tau * u2 - Gamma * u3 * (1.0 / 3.0) + 0.25 * u4;
         if (phi_out) phi_out[g] = x;
-        return (T)(Tau * u + Gamma * u2 - u3);
+        if constexpr (sizeof(T) == 4) { // FP32 fields: pointwise math in FP32, the sum in FP64
+            const float u = x, u2 = u * u, u3 = u2 * u, tau = (float)Tau, gam = (float)Gamma;
+            acc[0] += (double)(-0.5f * tau * u2 - gam * u3 * (1.0f / 3.0f) + 0.25f * u2 * u2);
+            return tau * u + gam * u2 - u3;
+        } else {
+            const double u = (double)x, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+            acc[0] += -0.5 * Tau * u2 - Gamma * u3 * (1.0 / 3.0) + 0.25 * u4;
+            return (T)(Tau * u + Gamma * u2 - u3);
+        }
     }
 };
 
@@ -309,6 +315,23 @@ template <typename T, int N> struct SOpGradFlowStep {
         const int a = kfold(i, N), b = kfold(j, N);
         const int n2 = a * a + b * b + kz * kz;
         const C2<T> gd = GD[n2]; // gathered here: fetching all E entries up front costs occupancy
+        if constexpr (sizeof(T) == 4) { // FP32 fields: the update in FP32, the three sums in FP64
+            const float g = gd.x, d = gd.y, w = (kz == 0 || kz == N / 2) ? 1.0f : 2.0f;
+            const float fr = v.x * (float)sc, fi = v.y * (float)sc, ur = u.x, ui = u.y, dtf = (float)dt;
+            float nr = (dtf * fr + ur) * d, ni = (dtf * fi + ui) * d;
+            if (n2 == 0) {
+                nr = 0.0f;
+                ni = 0.0f;
+            } else {
+                const float dr = g * ur - fr, di = g * ui - fi;
+                acc[1] += (double)(w * (dr * dr + di * di));
+            }
+            acc[0] += (double)(w * g * (ur * ur + ui * ui));
+            acc[2] = fmax(acc[2], (double)fabsf(nr - ur));
+            U[s] = C2<T>{nr, ni};
+            v = C2<T>{nr, ni};
+            return;
+        }
         const double g = (double)gd.x, d = (double)gd.y, w = hweight<N>(kz);
         const double fr = (double)v.x * sc, fi = (double)v.y * sc;
         const double ur = (double)u.x, ui = (double)u.y;
