@@ -39,8 +39,10 @@ struct Ctl {
 
 template <typename T, int N, int TPP_ = DefTPP<N>::value> struct Geo {
     static constexpr int TPP = TPP_, E = N / TPP_;
-    // column passes: B adjacent columns per block -> 128-byte row segments
-    static constexpr int B0 = 128 / (int)sizeof(C2<T>);
+    // column passes: B adjacent columns per block (row segments of 256 B in
+    // FP64, 128 B in FP32; measured faster than 128 B / 64 B in FP64 and than
+    // 32 columns in either precision)
+    static constexpr int B0 = 16;
     static constexpr int B = (64 / TPP > B0) ? 64 / TPP : B0;
     static constexpr int COL_THREADS = B * TPP;
     // FP64 column passes exchange real and imaginary parts separately through
