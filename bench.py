@@ -137,9 +137,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 512 MiB device copy (calibrates HBM byte counters)")
-    ap.add_argument("--shape", type=int, default=-1,
-                    help="FFT pass shape (threads per pencil of the three passes); -1: measure them all "
-                         "first and use the fastest (the tuner)")
+    ap.add_argument("--shape", default="-1",
+                    help="FFT pass shape index (threads per pencil of the three passes), or FP64,FP32 indices; "
+                         "-1: measure them all first and use the fastest (the tuner)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests of the serving stream (p50/p99 latency); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=12.0)
@@ -164,9 +164,11 @@ def main():
         del a, b
     shapes = spm.shapes(N)
     tuned = {}
+    want = [int(x) for x in str(args.shape).split(",")]
+    want = want * 2 if len(want) == 1 else want
     for prec in (spm.FP64, spm.FP32):
-        if args.shape >= 0:
-            tuned[prec] = (args.shape, None)
+        if want[prec] >= 0:
+            tuned[prec] = (want[prec], None)
         else:  # the tuner: every shape timed on 30 graph-replayed steps of a zero field
             best, us = spm.tune(N, prec, nmem=1, steps=30)
             tuned[prec] = (best, us)
