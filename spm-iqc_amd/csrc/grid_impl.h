@@ -142,6 +142,15 @@ struct GridImpl final : GridIface {
         rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)w}, nullptr);
     }
 
+    void gconv(Ctx &c, const void *in, void *out) override
+    {
+        rows<false, true>(c, in, c.W, OpNone{}, nullptr);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)out}, nullptr);
+    }
+
     void force(Ctx &c, void *x, const void *v, const double *s, double coef) override
     {
         rows<false, true>(c, x, c.W, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr);

@@ -74,6 +74,8 @@ struct GridIface {
     virtual void ngrad(Ctx &c, const void *phi, void *grad) = 0; // scal[SC_GN2] = nrmp^2
     virtual void hv(Ctx &c, const void *phi, const void *v, void *hv) = 0;
     virtual void precond(Ctx &c, void *w) = 0;
+    // out = IFFT(G FFT(in) / N^3) with the DC set to zero (the non-local part of ngrad_cam)
+    virtual void gconv(Ctx &c, const void *in, void *out) = 0;
     // x <- IFFT(FFT(x + nln(x) - coef * (*s) * v) / N^3 / (1 + G)), DC = 0
     // (one force step of SPM.cpp:34-99 / 103-168; s a device scalar)
     virtual void force(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;

@@ -14,21 +14,7 @@ namespace {
 
 constexpr int DOTB = 512;
 
-// part[b] = block partial of sum_r a[r] b[r]
-template <typename T>
-__global__ void __launch_bounds__(256) k_dot2(const T *__restrict__ a, const T *__restrict__ b, long n,
-                                             double *__restrict__ part)
-{
-    __shared__ double red[16];
-    double s = 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        s += (double)a[i] * (double)b[i];
-    double v[1] = {s};
-    block_reduce<1, 0u>(v, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
-}
-
-// out = sum(part) / R   (inpfp's weight, BasicOperators.cpp:132-143)
+// out = sum(part) / R   (inpfp's weight, BasicOperators.cpp:132-143; nrmp^2 of get_Tv)
 __global__ void __launch_bounds__(256) k_dot2_fin(const double *__restrict__ part, int nb, double R, double *out)
 {
     __shared__ double red[16];
@@ -37,6 +23,42 @@ __global__ void __launch_bounds__(256) k_dot2_fin(const double *__restrict__ par
     double v[1] = {s};
     block_reduce<1, 0u>(v, red);
     if (threadIdx.x == 0) *out = v[0] / R;
+}
+
+// Sums for <f, v> without forming f = ngrad_cam(x) (LPsrc.cpp:147-169):
+// f = -(G*x) + nl(x) - mean(nl), nl = Tau x + Gamma x^2 - x^3, and
+// <G*x, v> = <x, G*v> for the even kernel G (Parseval), so with vt = G*v
+// <f, v> = -mean(x vt) + mean(nl v) - mean(nl) mean(v).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fv_sums(const T *__restrict__ x, const T *__restrict__ v,
+                                                const T *__restrict__ vt, long n, double Tau, double Gamma,
+                                                double *__restrict__ part)
+{
+    __shared__ double red[4 * 16];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double u = (double)x[i], vi = (double)v[i];
+        const double nl = Tau * u + Gamma * u * u - u * u * u;
+        s[0] += u * (double)vt[i];
+        s[1] += nl * vi;
+        s[2] += nl;
+        s[3] += vi;
+    }
+    block_reduce<4, 0u>(s, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[(long)q * gridDim.x + blockIdx.x] = s[q];
+}
+
+__global__ void __launch_bounds__(256) k_fv_fin(const double *__restrict__ part, int nb, double R, double *out)
+{
+    __shared__ double red[4 * 16];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] += part[(long)q * nb + b];
+    block_reduce<4, 0u>(s, red);
+    if (threadIdx.x == 0) *out = (-s[0] + s[1]) / R - (s[2] / R) * (s[3] / R);
 }
 
 // get_Tv (SPM.cpp:15-32), in two kernels: v = Phi1 - Phi0 (with its norm), v /= nrmp(v)
@@ -94,20 +116,20 @@ int get_tv(spm_ctx *h, const void *p0, const void *p1, void *v)
     return h->c.prec == SPM_FP64 ? launch_tv<double>(h, p0, p1, v) : launch_tv<float>(h, p0, p1, v);
 }
 
-// Natural_Forces (coef 1) / MinMax_Forces (coef 2) on x in place.
-// f = ngrad_cam(x) lands in fbuf.
-int forces(spm_ctx *h, void *x, const void *v, double coef, void *fbuf)
+// Natural_Forces (coef 1) / MinMax_Forces (coef 2) on x in place, given the
+// tangent v and vt = G*v (one G convolution per tangent serves every force
+// with that tangent).
+int forces(spm_ctx *h, void *x, const void *v, const void *vt, double coef)
 {
     Ctx &c = h->c;
-    c.g->ngrad(c, x, fbuf);
     double *part = c.slot(6);
     if (c.prec == SPM_FP64)
-        hipLaunchKernelGGL(k_dot2<double>, dim3(DOTB), dim3(256), 0, c.stream, (const double *)fbuf,
-                           (const double *)v, c.R, part);
+        hipLaunchKernelGGL(k_fv_sums<double>, dim3(DOTB), dim3(256), 0, c.stream, (const double *)x,
+                           (const double *)v, (const double *)vt, c.R, c.Tau, c.Gamma, part);
     else
-        hipLaunchKernelGGL(k_dot2<float>, dim3(DOTB), dim3(256), 0, c.stream, (const float *)fbuf,
-                           (const float *)v, c.R, part);
-    hipLaunchKernelGGL(k_dot2_fin, dim3(1), dim3(256), 0, c.stream, part, DOTB, (double)c.R, c.scal + SC_FV);
+        hipLaunchKernelGGL(k_fv_sums<float>, dim3(DOTB), dim3(256), 0, c.stream, (const float *)x,
+                           (const float *)v, (const float *)vt, c.R, c.Tau, c.Gamma, part);
+    hipLaunchKernelGGL(k_fv_fin, dim3(1), dim3(256), 0, c.stream, part, DOTB, (double)c.R, c.scal + SC_FV);
     c.g->force(c, x, v, c.scal + SC_FV, coef);
     return spm_launch_status("forces");
 }
@@ -156,7 +178,8 @@ int spm_natural_force(spm_ctx *h, void *x, const void *v)
     CHECK_CTX(h);
     if (!x || !v || x == v) return spm_fail(SPM_EINVAL, "spm_natural_force: bad fields");
     RCHK(ensure_fields(h, 1));
-    return forces(h, x, v, 1.0, h->ws);
+    c.g->gconv(c, v, h->ws);
+    return forces(h, x, v, h->ws, 1.0);
 }
 
 int spm_minmax_force(spm_ctx *h, void *x, const void *v)
@@ -164,7 +187,8 @@ int spm_minmax_force(spm_ctx *h, void *x, const void *v)
     CHECK_CTX(h);
     if (!x || !v || x == v) return spm_fail(SPM_EINVAL, "spm_minmax_force: bad fields");
     RCHK(ensure_fields(h, 1));
-    return forces(h, x, v, 2.0, h->ws);
+    c.g->gconv(c, v, h->ws);
+    return forces(h, x, v, h->ws, 2.0);
 }
 
 int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double tol, int *iters, double *res_out,
@@ -173,9 +197,10 @@ int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double t
     CHECK_CTX(h);
     if (!phi0 || !phi1 || phi0 == phi1) return spm_fail(SPM_EINVAL, "spm_spring_pairing: bad replicas");
     if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_spring_pairing: maxiter >= 0");
-    RCHK(ensure_fields(h, 2));
+    RCHK(ensure_fields(h, 3));
     void *tv = h->ws;
     void *f = (char *)h->ws + (size_t)c.R * c.esz;
+    void *vt = (char *)h->ws + 2 * (size_t)c.R * c.esz; // G * tv
     void *Phi[2] = {phi0, phi1};
     const double dt = 0.001, ks = 1; // SPM.cpp:177-178
     double res = 1e-6;               // SPM.cpp:180
@@ -183,11 +208,13 @@ int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double t
     while (res > tol && iter < maxiter) {
         iter = iter + 1;
         RCHK(get_tv(h, Phi[0], Phi[1], tv));
-        for (int i = 0; i < 2; ++i) RCHK(forces(h, Phi[i], tv, 1.0, f));
+        c.g->gconv(c, tv, vt);
+        for (int i = 0; i < 2; ++i) RCHK(forces(h, Phi[i], tv, vt, 1.0));
         RCHK(spring(h, Phi[0], Phi[1], dt, ks));
         RCHK(get_tv(h, Phi[0], Phi[1], tv));
+        c.g->gconv(c, tv, vt);
         for (int i = 0; i < 2; ++i) {
-            RCHK(forces(h, Phi[i], tv, 2.0, f));
+            RCHK(forces(h, Phi[i], tv, vt, 2.0));
             c.g->ngrad(c, Phi[i], f);
             hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, c.stream, c.scal + SC_GN2, c.scal + SC_RES0 + i);
         }
