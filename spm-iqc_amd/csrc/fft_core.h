@@ -185,6 +185,12 @@ SPM_DEFTPP(192, 16)
 SPM_DEFTPP(256, 16)
 #undef SPM_DEFTPP
 
+// LDS slot of twiddle e in a stage's table: one spare slot after every 8, so
+// the strided reads of the later stages (entries 4 q jm apart across lanes)
+// fall in different banks.
+__device__ __forceinline__ constexpr int twpad(int e) { return e + (e >> 3); }
+template <int N> constexpr int twiddle_slots() { return N + N / 8; }
+
 // LDS index of pencil element e.  PAD inserts one slot after every 16
 // elements: the first stage writes element t*R + q from thread t, which
 // without padding puts every thread of a pencil in the same bank.
@@ -211,7 +217,7 @@ __device__ __forceinline__ void stockham_stage(C2<T> (&v)[N / TPP], C2<T> *lds, 
         if constexpr (NS > 1) {
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                C2<T> w = tw[jm * q * (N / (NS * R))];
+                C2<T> w = tw[twpad(jm * q * (N / (NS * R)))];
                 if constexpr (DIR > 0) w.y = -w.y;
                 a[q] = cmul(a[q], w);
             }
@@ -235,8 +241,8 @@ __device__ __forceinline__ void stockham_stage(C2<T> (&v)[N / TPP], C2<T> *lds, 
 
 // The same stage with the exchange through a real-valued LDS area (half the
 // bytes, so twice the blocks fit on a CU): real parts first, then imaginary
-// parts, which wait in registers meanwhile.  No padding (column passes).
-template <typename T, int N, int TPP, int DIR, int R, int NS, bool LAST, int LS>
+// parts, which wait in registers meanwhile.
+template <typename T, int N, int TPP, int DIR, int R, int NS, bool LAST, int LS, bool PAD>
 __device__ __forceinline__ void stockham_stage_split(C2<T> (&v)[N / TPP], T *lds, const C2<T> *__restrict__ tw,
                                                      int t)
 {
@@ -254,7 +260,7 @@ __device__ __forceinline__ void stockham_stage_split(C2<T> (&v)[N / TPP], T *lds
         if constexpr (NS > 1) {
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                C2<T> w = tw[jm * q * (N / (NS * R))];
+                C2<T> w = tw[twpad(jm * q * (N / (NS * R)))];
                 if constexpr (DIR > 0) w.y = -w.y;
                 a[q] = cmul(a[q], w);
             }
@@ -267,7 +273,7 @@ __device__ __forceinline__ void stockham_stage_split(C2<T> (&v)[N / TPP], T *lds
             const int base = (j / NS) * NS * R + jm;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                lds[(base + q * NS) * LS] = a[q].x;
+                lds[lpad<PAD>(base + q * NS) * LS] = a[q].x;
                 hold[p * R + q] = a[q].y;
             }
         }
@@ -275,7 +281,7 @@ __device__ __forceinline__ void stockham_stage_split(C2<T> (&v)[N / TPP], T *lds
     if constexpr (!LAST) {
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m].x = lds[(t + TPP * m) * LS];
+        for (int m = 0; m < E; ++m) v[m].x = lds[lpad<PAD>(t + TPP * m) * LS];
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < NB; ++p) {
@@ -283,32 +289,32 @@ __device__ __forceinline__ void stockham_stage_split(C2<T> (&v)[N / TPP], T *lds
             const int jm = (NS > 1) ? j % NS : 0;
             const int base = (j / NS) * NS * R + jm;
 #pragma unroll
-            for (int q = 0; q < R; ++q) lds[(base + q * NS) * LS] = hold[p * R + q];
+            for (int q = 0; q < R; ++q) lds[lpad<PAD>(base + q * NS) * LS] = hold[p * R + q];
         }
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m].y = lds[(t + TPP * m) * LS];
+        for (int m = 0; m < E; ++m) v[m].y = lds[lpad<PAD>(t + TPP * m) * LS];
     }
 }
 
-template <typename T, int N, int TPP, int DIR, int LS>
+template <typename T, int N, int TPP, int DIR, int LS, bool PAD = false>
 __device__ __forceinline__ void fft_pencil_split(C2<T> (&v)[N / TPP], T *lds, const C2<T> *__restrict__ tw, int t)
 {
     using P = FftPlan<N, TPP>;
     if constexpr (P::NST == 1) {
-        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, true, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, true, LS, PAD>(v, lds, tw, t);
     } else if constexpr (P::NST == 2) {
-        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
-        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, true, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, true, LS, PAD>(v, lds, tw, t);
     } else if constexpr (P::NST == 3) {
-        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
-        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, false, LS>(v, lds, tw, t);
-        stockham_stage_split<T, N, TPP, DIR, P::R2, P::R0 * P::R1, true, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R2, P::R0 * P::R1, true, LS, PAD>(v, lds, tw, t);
     } else {
-        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS>(v, lds, tw, t);
-        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, false, LS>(v, lds, tw, t);
-        stockham_stage_split<T, N, TPP, DIR, P::R2, P::R0 * P::R1, false, LS>(v, lds, tw, t);
-        stockham_stage_split<T, N, TPP, DIR, P::R3, P::R0 * P::R1 * P::R2, true, LS>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R0, 1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R1, P::R0, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R2, P::R0 * P::R1, false, LS, PAD>(v, lds, tw, t);
+        stockham_stage_split<T, N, TPP, DIR, P::R3, P::R0 * P::R1 * P::R2, true, LS, PAD>(v, lds, tw, t);
     }
 }
 

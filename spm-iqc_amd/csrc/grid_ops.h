@@ -463,20 +463,24 @@ __device__ __forceinline__ void write_partials(double (&acc)[NRED > 0 ? NRED : 1
     }
 }
 
-// Twiddle table exp(-2 pi i e / N) into LDS.  The first Stockham stage uses no
+// Twiddle table exp(-2 pi i e / N) into LDS (padded, twpad).  The first Stockham stage uses no
 // twiddles and ends in a barrier, so no barrier is needed here.
 template <typename T, int N>
 __device__ __forceinline__ void load_twiddles(C2<T> *stw, const C2<T> *__restrict__ tw)
 {
-    for (int e = threadIdx.x; e < N; e += blockDim.x) stw[e] = tw[e];
+    for (int e = threadIdx.x; e < N; e += blockDim.x) stw[twpad(e)] = tw[e];
 }
 
 // Row (k-axis) pass over npairs pairs of rows.
 //   IN_SPEC:  input rows are half spectra (row stride NHP): c2r (unnormalised)
 //   !IN_SPEC: input rows are real (row stride N)
 //   OUT_SPEC: forward r2c (unnormalised) of op(x) into out (row stride NHP)
-// The block's input rows are contiguous in memory: they are staged through
-// LDS with coalesced loads, each element read once.
+// Half-spectrum input is read straight into the pencil layout (each element
+// by the threads that need it; the second read of an element hits the cache);
+// real input rows are contiguous and staged through LDS with 16-byte loads.
+// Stockham exchanges and the final separation of the two packed rows go
+// through LDS as real parts, then imaginary parts (half the LDS of complex
+// exchanges).
 template <typename T, int N, int TPP_, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
 __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
     k_rows(const void *in, C2<T> *out, int NHP, long npairs, const C2<T> *__restrict__ tw, Op op0,
@@ -484,9 +488,9 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
 {
     using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, PP = G::PP, NH = N / 2 + 1;
-    constexpr int XCH = PP * G::RSTR, STG = IN_SPEC ? 2 * PP * NH : PP * N;
-    __shared__ C2<T> lds[XCH > STG ? XCH : STG];
-    __shared__ C2<T> stw[N];
+    constexpr int XCH = PP * G::RSTR, STG = IN_SPEC ? 0 : 2 * PP * N; // in reals
+    __shared__ __align__(16) T lds[XCH > STG ? XCH : STG];
+    __shared__ C2<T> stw[twiddle_slots<N>()];
     __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
     const int mb = blockIdx.y; // batch member
     if (ctl && ctl[mb].stop) return;
@@ -507,25 +511,19 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
     const long ra = 2 * pair, rb = ra + 1;
     const long row0 = 2L * PP * blockIdx.x;
     const long nrows = 2 * npairs;
-    C2<T> *my = lds + pp * G::RSTR;
+    T *my = lds + pp * G::RSTR;
     C2<T> v[E];
     double acc[Op::NRED > 0 ? Op::NRED : 1];
 #pragma unroll
     for (int q = 0; q < (Op::NRED > 0 ? Op::NRED : 1); ++q) acc[q] = 0.0;
 
     if constexpr (IN_SPEC) {
-        const C2<T> *S = (const C2<T> *)in;
-        for (int idx = threadIdx.x; idx < 2 * PP * NH; idx += blockDim.x) {
-            const int r = idx / NH, k = idx - r * NH;
-            if (row0 + r < nrows) lds[idx] = S[(row0 + r) * NHP + k];
-        }
-        __syncthreads();
-        const C2<T> *la = lds + (2 * pp) * NH, *lb = la + NH;
+        const C2<T> *la = (const C2<T> *)in + ra * NHP, *lb = la + NHP;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = t + TPP * m;
             const int q = p <= N / 2 ? p : N - p;
-            C2<T> a = la[q], b = lb[q];
+            C2<T> a = active ? la[q] : C2<T>{0, 0}, b = active ? lb[q] : C2<T>{0, 0};
             if (q == 0 || q == N / 2) { // c2r reads only the real part of DC and Nyquist
                 a.y = 0;
                 b.y = 0;
@@ -533,7 +531,7 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
             // Z = Xa + i Xb on 0..N/2, conj(Xa) + i conj(Xb) above
             v[m] = p <= N / 2 ? C2<T>{a.x - b.y, a.y + b.x} : C2<T>{a.x + b.y, b.x - a.y};
         }
-        fft_pencil<T, N, TPP, +1, 1, true>(v, my, stw, t);
+        fft_pencil_split<T, N, TPP, +1, 1, true>(v, my, stw, t);
     } else {
         // 16-byte chunks of the block's 2*PP contiguous real rows
         constexpr int VPC = 16 / sizeof(T);
@@ -545,7 +543,7 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
         for (int c = threadIdx.x; c < 2 * PP * N / VPC; c += blockDim.x)
             if ((long)c * VPC < nval) dstl[c] = src[c];
         __syncthreads();
-        const T *lr = (const T *)lds + (2 * pp) * N;
+        const T *lr = lds + (2 * pp) * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int p = t + TPP * m;
@@ -561,21 +559,34 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
         }
     }
     if constexpr (OUT_SPEC) {
-        fft_pencil<T, N, TPP, -1, 1, true>(v, my, stw, t);
+        fft_pencil_split<T, N, TPP, -1, 1, true>(v, my, stw, t);
+        // separation of the two rows: needs Z[k] and Z[N - k] (another thread's)
+        constexpr int EH = E / 2 + 1;
+        T ax[EH], bx[EH];
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m];
+        for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].x;
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < EH; ++m) {
+            const int k = t + TPP * m;
+            ax[m] = k <= N / 2 ? my[k] : (T)0;
+            bx[m] = k <= N / 2 ? my[(N - k) % N] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].y;
         __syncthreads();
         if (active) {
 #pragma unroll
-            for (int m = 0; m <= E / 2; ++m) {
+            for (int m = 0; m < EH; ++m) {
                 const int k = t + TPP * m;
                 if (k <= N / 2) {
-                    const C2<T> A = my[k], Bc = my[(N - k) % N];
-                    // B = conj(Z[N-k]); Xa = (A + B)/2, Xb = -i (A - B)/2
+                    const T ay = my[k], by = my[(N - k) % N];
+                    // A = Z[k], B = Z[N-k]: Xa = (A + conj B)/2, Xb = -i (A - conj B)/2
                     const T hx = (T)0.5;
-                    out[ra * NHP + k] = C2<T>{hx * (A.x + Bc.x), hx * (A.y - Bc.y)};
-                    out[rb * NHP + k] = C2<T>{hx * (A.y + Bc.y), hx * (Bc.x - A.x)};
+                    out[ra * NHP + k] = C2<T>{hx * (ax[m] + bx[m]), hx * (ay - by)};
+                    out[rb * NHP + k] = C2<T>{hx * (ay + by), hx * (bx[m] - ax[m])};
                 }
             }
         }
@@ -593,7 +604,7 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::ROW_THREADS))
     using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, PP = G::PP;
     __shared__ C2<T> lds[PP * G::RSTR];
-    __shared__ C2<T> stw[N];
+    __shared__ C2<T> stw[twiddle_slots<N>()];
     load_twiddles<T, N>(stw, tw);
     const int t = threadIdx.x % TPP, pp = threadIdx.x / TPP;
     const long row = (long)blockIdx.x * PP + pp;
@@ -621,7 +632,7 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
     using G = Geo<T, N, TPP_>;
     constexpr int TPP = G::TPP, E = G::E, B = G::B;
     __shared__ typename G::XT lds[N * B];
-    __shared__ C2<T> stw[N];
+    __shared__ C2<T> stw[twiddle_slots<N>()];
     __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
     const int mb = blockIdx.y; // batch member
     if (ctl && ctl[mb].stop) return;
