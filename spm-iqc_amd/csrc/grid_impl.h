@@ -193,8 +193,9 @@ struct GridImpl final : GridIface {
         const int kf = (int)std::min<long>(64, c.nb_j);
         const int nm = c.nmem;
         const long bs = c.C;
+        const bool nt = 2.0 * (double)bs * sizeof(CT) * nm > 200e6; // W + U beyond the 256 MB last-level cache
         cols_i<true, true, true>(c, c.W, c.W,
-                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs},
+                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
                                  c.slot(3), c.ctl, -1, -1, nm, bs);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl, -1, -1,
                                   ProGradFlowReduce{c.slot(3), c.nb_i, c.slot(2), c.nb_rows, c.slot(4), kf}, nm, bs);

@@ -15,12 +15,14 @@ template <typename T> static GridIface *make(int shape)
     case 5: return new GridImpl<T, 256, 32, 32, 16>();
     case 6: return new GridImpl<T, 256, 32, 8, 16>();
     case 7: return new GridImpl<T, 256, 64, 32, 16>();
+    case 8: return new GridImpl<T, 256, 16, 32, 16>();
+    case 9: return new GridImpl<T, 256, 16, 8, 16>();
     default: return new GridImpl<T, 256>();
     }
 }
 } // namespace
 
-const int kShapes256[][3] = {{16, 16, 16}, {32, 16, 32}, {16, 16, 32}, {32, 16, 16}, {64, 16, 16}, {32, 32, 16}, {32, 8, 16}, {64, 32, 16}};
+const int kShapes256[][3] = {{16, 16, 16}, {32, 16, 32}, {16, 16, 32}, {32, 16, 16}, {64, 16, 16}, {32, 32, 16}, {32, 8, 16}, {64, 32, 16}, {16, 32, 16}, {16, 8, 16}};
 
 GridIface *make_grid_n256(int prec, int shape) { return prec ? make<float>(shape) : make<double>(shape); }
 

@@ -305,13 +305,37 @@ template <typename T, int N> struct SOpGradFlowStep {
     const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
     long ubs; // batched: member stride of U
+    // U is read and written with non-temporal hints: set when U and W together
+    // overflow the last-level cache, so W (read again by the next three passes)
+    // keeps it; when both fit, the hints only cost
+    bool nt;
     __device__ __forceinline__ SOpGradFlowStep bind(int m) const
     {
         SOpGradFlowStep o = *this;
         o.U = U + (long)m * ubs;
         return o;
     }
-    __device__ __forceinline__ const C2<T> *pre_src() const { return U; }
+    __device__ __forceinline__ C2<T> load_pre(long s) const
+    {
+        if (nt) {
+            typedef T vt2 __attribute__((ext_vector_type(2)));
+            const vt2 w = __builtin_nontemporal_load((const vt2 *)(U + s));
+            return C2<T>{w.x, w.y};
+        }
+        return U[s];
+    }
+    __device__ __forceinline__ void store_u(long s, C2<T> x) const
+    {
+        if (nt) {
+            typedef T vt2 __attribute__((ext_vector_type(2)));
+            vt2 w;
+            w.x = x.x;
+            w.y = x.y;
+            __builtin_nontemporal_store(w, (vt2 *)(U + s));
+        } else {
+            U[s] = x;
+        }
+    }
     __device__ __forceinline__ void operator()(C2<T> &v, C2<T> u, long s, int i, int j, int kz, double *acc) const
     {
         const int a = kfold(i, N), b = kfold(j, N);
@@ -330,7 +354,7 @@ template <typename T, int N> struct SOpGradFlowStep {
             }
             acc[0] += (double)(w * g * (ur * ur + ui * ui));
             acc[2] = fmax(acc[2], (double)fabsf(nr - ur));
-            U[s] = C2<T>{nr, ni};
+            store_u(s, C2<T>{nr, ni});
             v = C2<T>{nr, ni};
             return;
         }
@@ -348,7 +372,7 @@ template <typename T, int N> struct SOpGradFlowStep {
         acc[0] += w * g * (ur * ur + ui * ui);
         const C2<T> un = {(T)nr, (T)ni};
         acc[2] = fmax(acc[2], fabs((double)un.x - ur));
-        U[s] = un;
+        store_u(s, un);
         v = un;
     }
 };
@@ -659,9 +683,8 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = active ? src[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
     if constexpr (SOp::PRELOAD) {
-        const C2<T> *ps = op.pre_src();
 #pragma unroll
-        for (int m = 0; m < E; ++m) pre[m] = active ? ps[off + (long)(t + TPP * m) * pstride] : C2<T>{0, 0};
+        for (int m = 0; m < E; ++m) pre[m] = active ? op.load_pre(off + (long)(t + TPP * m) * pstride) : C2<T>{0, 0};
     }
     if constexpr (FWD) {
         if constexpr (G::XSPLIT)
