@@ -37,7 +37,7 @@ class SpmError(RuntimeError):
 class _Params(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("precision", ctypes.c_int), ("L", ctypes.c_double),
                 ("kscale", ctypes.c_double), ("Tau", ctypes.c_double), ("Gamma", ctypes.c_double),
-                ("dt", ctypes.c_double), ("shape", ctypes.c_int)]
+                ("dt", ctypes.c_double), ("shape", ctypes.c_int), ("step_passes", ctypes.c_int)]
 
 
 class _Layout(ctypes.Structure):
@@ -137,8 +137,9 @@ def _torch():
     return torch
 
 
-def _params(N, precision, L, kscale, Tau, Gamma, dt, shape):
-    return _Params(int(N), int(precision), float(L), float(kscale), float(Tau), float(Gamma), float(dt), int(shape))
+def _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes=0):
+    return _Params(int(N), int(precision), float(L), float(kscale), float(Tau), float(Gamma), float(dt), int(shape),
+                   int(step_passes))
 
 
 def shapes(N):
@@ -170,10 +171,11 @@ class Grid:
     complex.  Mirrors the reference's process-global state (Data.h:4-28).
     """
 
-    def __init__(self, N, precision=FP64, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, shape=0):
+    def __init__(self, N, precision=FP64, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, shape=0,
+                 step_passes=0):
         torch = _torch()
         self.torch = torch
-        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape)
+        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes)
         h = _c()
         _check(lib().spm_create(ctypes.byref(p), ctypes.byref(h)), "spm_create")
         self.h = h
@@ -415,10 +417,10 @@ class Batch:
     torch tensors on the GPU in the batch precision."""
 
     def __init__(self, N, precision=FP64, capacity=8, hist_cap=4096, L=26.0, kscale=1.0, Tau=-0.0061,
-                 Gamma=0.30, dt=0.3, shape=0):
+                 Gamma=0.30, dt=0.3, shape=0, step_passes=0):
         torch = _torch()
         self.torch = torch
-        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape)
+        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes)
         h = _c()
         _check(lib().spm_batch_create(ctypes.byref(p), int(capacity), int(hist_cap), ctypes.byref(h)),
                "spm_batch_create")

@@ -1,0 +1,493 @@
+// gf_split.h -- the gradient-flow step of a 256^3 grid in two passes over the
+// work array instead of four (gfx950, HIP).
+//
+// The length-256 transform along j is split by one Cooley-Tukey step,
+// j = 16 h + l:
+//   stage A: 16-point DFTs over h for each l (rows 16 apart in one i plane),
+//            then the twiddle exp(-2 pi i l k1 / 256); digit k1 is stored at h;
+//   stage B: 16-point DFTs over l for each k1 (16 adjacent rows); digit k2 is
+//            stored at l, so position (h, l) holds frequency J = h + 16 l.
+// Stage A rides in the row (kz) pass and stage B in the i pass, so one
+// iteration of get_min_u_F (LPsrc.cpp:52-111) is
+//   pass 1 (rows):    A^-1, c2r, f(phi) and the density sum, r2c, A     W in/out
+//   pass 2 (columns): B, i FFT, update with U and its sums, i FFT^-1, B^-1
+//                                                          W in/out, U in/out
+// six array passes per step instead of the ten of the axis-by-axis passes.
+//
+// W and U of one field are stored as [i][h][ko][l][ki], kz = KT ko + ki, with
+// KT = 2 (FP64) or 4 (FP32).  A pass-1 block reads, per (h, ko), the 64-byte
+// half of a 128-byte line (2 l x KT ki); a pass-2 block reads, per i, every
+// other 16 bytes of 512 (16 l x KT/2 ki).  The two blocks sharing those lines
+// are launched 8 block indices apart, i.e. on the same XCD at the same time,
+// so each line comes from HBM once.  Entries with kz > N/2 are padding.
+#pragma once
+#include "grid_ops.h"
+
+namespace spm {
+
+template <typename T, int N> struct SJ {
+    static_assert(N == 256, "two-pass gradient-flow step: N = 256");
+    static constexpr int NH = N / 2 + 1;                // valid kz
+    static constexpr int DH = 16, DL = N / DH;          // ranges of the digits h and l
+    static constexpr int KT = sizeof(T) == 8 ? 2 : 4;   // kz per contiguous group
+    static constexpr int KO = (NH + KT - 1) / KT;       // kz groups
+    static constexpr long HSTR = (long)KO * DL * KT;    // stride of h
+    static constexpr long ISTR = DH * HSTR;             // stride of i
+    static constexpr long CB = (long)N * ISTR;          // complex entries per field (W and U)
+    __host__ __device__ static constexpr long wof(int i, int h, int ko, int l, int ki)
+    {
+        return (long)i * ISTR + (long)h * HSTR + ((long)ko * DL + l) * KT + ki;
+    }
+    static constexpr int TPP = 16, E = N / TPP;
+    // pass 1: S1 = 2 l of one i (16 S1 rows, 8 S1 pairs of TPP threads); block b
+    // and b + 8 take l pairs 2 lp and 2 lp + 1 (one line)
+    static constexpr int S1 = 2;
+    static constexpr int NP = DH * S1 / 2;
+    static constexpr int R_THREADS = NP * TPP;
+    static constexpr long R_BLOCKS = (long)N * (DL / S1);
+    __device__ static void rblock(int b, int &i, int &lo)
+    {
+        const int x = b % 8, s = (b / 8) % 2, q = b / 16;
+        const int P = q * 8 + x; // (i, lp), lp < DL / (2 S1)
+        i = P / (DL / (2 * S1));
+        lo = 2 * (P % (DL / (2 * S1))) + s;
+    }
+    // pass 2: (h, ko, half kg of the ki): KB = KT/2 ki, NC = DL KB columns (l, ka);
+    // block b and b + 8 take kg = 0 and 1 of one (h, ko)
+    static constexpr int KB = KT / 2;
+    static constexpr int NC = DL * KB;
+    static constexpr int C_THREADS = NC * TPP;
+    static constexpr long C_BLOCKS = 2L * DH * KO;
+    __device__ static void cblock(int b, int &h, int &ko, int &kg)
+    {
+        const int x = b % 8, s = (b / 8) % 2, q = b / 16;
+        const int P = q * 8 + x; // (h, ko)
+        h = P / KO;
+        ko = P % KO;
+        kg = s;
+    }
+    // U: the state of one pass-2 block is one contiguous chunk [i][l][ka]
+    __host__ __device__ static constexpr long uof(int h, int ko, int kg, int i, int c)
+    {
+        return ((((long)h * KO + ko) * 2 + kg) * N + i) * NC + c;
+    }
+    static_assert(R_THREADS == S1 * (N / 2), "pass 1: one (l, kz < N/2) column per thread");
+    static_assert(C_THREADS == N * KB, "pass 2: one (i, ka) line per thread");
+    static_assert((DH * KO) % 8 == 0 && (N * DL / (2 * S1)) % 8 == 0, "block maps");
+};
+
+// ----------------------------------------------------------------------
+// Block-0 prologues closing an iteration (partial sums are complete at the
+// kernel boundary).  E2 partials come from pass 1 of the previous iteration,
+// the E1 / ||grad F||^2 / residual partials from pass 2 of this one.
+// ----------------------------------------------------------------------
+
+// Pass 2, block 0: sum of the density partials of pass 1 into e2[m].
+struct ProSJDensity {
+    static constexpr bool NONE = false;
+    const double *pC;
+    long nC;
+    double *e2;
+    __device__ __forceinline__ ProSJDensity bind(int m) const
+    {
+        ProSJDensity o = *this;
+        o.pC = pC + (long)m * nC;
+        o.e2 = e2 + m;
+        return o;
+    }
+    __device__ int nblk() const { return 1; }
+    __device__ void operator()(double *red) const
+    {
+        double v[1] = {0.0};
+        for (long b = threadIdx.x; b < nC; b += blockDim.x) v[0] += pC[b];
+        block_reduce<1, 0u>(v, red);
+        if (threadIdx.x == 0) *e2 = v[0];
+        __syncthreads();
+    }
+};
+
+// Pass 1, block 0: E = E1 + E2, ||grad F||, residual into hist[3*iter ..],
+// iter += 1, and the stop test of LPsrc.cpp:52.
+struct ProSJFinalize {
+    static constexpr bool NONE = false;
+    const double *pA; // pass-2 partials: 3 values per block
+    long nA;
+    const double *e2;
+    double R;
+    Ctl *ctl;
+    double *hist;
+    long hstride;
+    __device__ __forceinline__ ProSJFinalize bind(int m) const
+    {
+        ProSJFinalize o = *this;
+        o.pA = pA + (long)m * 3 * nA;
+        o.e2 = e2 + m;
+        o.ctl = ctl + m;
+        o.hist = hist + (long)m * hstride;
+        return o;
+    }
+    __device__ int nblk() const { return 1; }
+    __device__ void operator()(double *red) const
+    {
+        double v[3] = {0.0, 0.0, 0.0};
+        for (long b = threadIdx.x; b < nA; b += blockDim.x) {
+            v[0] += pA[b];
+            v[1] += pA[nA + b];
+            v[2] = fmax(v[2], pA[2 * nA + b]);
+        }
+        block_reduce<3, 4u>(v, red);
+        if (threadIdx.x == 0) {
+            const double E = 0.5 * v[0] + *e2 / R, gn = sqrt(v[1]), res = v[2];
+            const int it = ctl->iter;
+            hist[3L * it + 0] = E;
+            hist[3L * it + 1] = gn;
+            hist[3L * it + 2] = res;
+            ctl->iter = it + 1;
+            if (!(res > ctl->res_tol) || (ctl->gn_tol > 0 && gn <= ctl->gn_tol) || it + 1 >= ctl->max_iter)
+                ctl->stop = 1;
+        }
+        __syncthreads();
+    }
+};
+
+// ----------------------------------------------------------------------
+// Pass 1: one i plane, the rows j = 16 h + l for all h and S1 consecutive l.
+//   IN_SPEC:  input W: A^-1, then c2r of each row
+//   !IN_SPEC: input real field (natural layout [i][j][k])
+//   OUT_SPEC: r2c of op(x) per row, then A, into W; otherwise op only
+// Rows pair up as (l even, l odd) of one h: the two real rows of a pair are
+// the real and imaginary parts of one complex length-N FFT.  The column role
+// of a thread (the 16 h of one (l, kz < N/2)) does stage A; the row role
+// (TPP threads per pair) does the row FFTs; the row buffer in LDS converts.
+// ----------------------------------------------------------------------
+template <typename T, int N, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
+__global__ void __launch_bounds__((SJ<T, N>::R_THREADS))
+    k_sj_rows(const void *in, void *out, const C2<T> *__restrict__ tw, Op op0, double *__restrict__ part,
+              const Ctl *ctl, Pro pro0, long in_bs, long out_bs)
+{
+    using S = SJ<T, N>;
+    constexpr int TPP = S::TPP, E = S::E, NP = S::NP, S1 = S::S1, DH = S::DH, KT = S::KT, NR = 2 * NP;
+    constexpr int RS = N / 2 + 8;                      // row stride of the row buffer (reals)
+    constexpr int RSTR = pencil_slots<N, true>() + 1;  // pencil exchange stride (reals)
+    constexpr int NL = NR * RS > NP * RSTR ? NR * RS : NP * RSTR;
+    __shared__ __align__(16) T lds[NL];
+    __shared__ T nyq[NR];            // kz = N/2 entry of each row (real)
+    __shared__ C2<T> nyin[DH * S1];  // kz = N/2 column input, before A^-1
+    __shared__ C2<T> stw[twiddle_slots<N>()];
+    __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
+    const int mb = blockIdx.y;
+    if (ctl && ctl[mb].stop) return;
+    const Op op = bind_member(op0, mb);
+    if constexpr (!Pro::NONE) {
+        const Pro pro = bind_member(pro0, mb);
+        if ((int)blockIdx.x < pro.nblk()) pro(red);
+    }
+    load_twiddles<T, N>(stw, tw);
+    int i, lo;
+    S::rblock(blockIdx.x, i, lo);
+    const int tid = threadIdx.x;
+    // column role: the 16 h values of (l = S1 lo + li, kz)
+    const int ki = tid % KT, li = (tid / KT) % S1, ko = tid / (KT * S1);
+    const int kz = ko * KT + ki, l = S1 * lo + li;
+    // row role: pair pp = rows ra, ra + 1 (row r = h S1 + li)
+    const int t = tid % TPP, pp = tid / TPP;
+    const int ra = 2 * pp, rb = ra + 1;
+    T *my = lds + pp * RSTR;
+    C2<T> v[E];
+    C2<T> c[DH];
+    double acc[Op::NRED > 0 ? Op::NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (Op::NRED > 0 ? Op::NRED : 1); ++q) acc[q] = 0.0;
+
+    if constexpr (IN_SPEC) {
+        const C2<T> *W = (const C2<T> *)in + mb * in_bs;
+#pragma unroll
+        for (int h = 0; h < DH; ++h) c[h] = W[S::wof(i, h, ko, l, ki)];
+        C2<T> ny = {0, 0};
+        if (tid < DH * S1) ny = W[S::wof(i, tid / S1, (N / 2) / KT, S1 * lo + tid % S1, (N / 2) % KT)];
+        __syncthreads(); // twiddles
+        // A^-1: conjugate twiddle, inverse 16-point DFT over h
+#pragma unroll
+        for (int h = 1; h < DH; ++h) {
+            C2<T> w = stw[twpad(l * h)];
+            w.y = -w.y;
+            c[h] = cmul(c[h], w);
+        }
+        dft<16, +1>(c);
+        // to the row buffer: real parts, then imaginary parts
+        T hold[DH];
+#pragma unroll
+        for (int h = 0; h < DH; ++h) {
+            lds[(h * S1 + li) * RS + kz] = c[h].x;
+            hold[h] = c[h].y;
+        }
+        if (tid < DH * S1) nyin[tid] = ny;
+        __syncthreads();
+        if (tid < S1) { // the kz = N/2 columns: only their real parts enter the c2r
+            const int ln = S1 * lo + tid;
+#pragma unroll
+            for (int h = 0; h < DH; ++h) {
+                c[h] = nyin[h * S1 + tid];
+                if (h > 0) {
+                    C2<T> w = stw[twpad(ln * h)];
+                    w.y = -w.y;
+                    c[h] = cmul(c[h], w);
+                }
+            }
+            dft<16, +1>(c);
+#pragma unroll
+            for (int h = 0; h < DH; ++h) nyq[h * S1 + tid] = c[h].x;
+        }
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPP * m;
+            const int q = p <= N / 2 ? p : N - p;
+            v[m].x = q < N / 2 ? lds[ra * RS + q] : (T)0;
+            v[m].y = q < N / 2 ? lds[rb * RS + q] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < DH; ++h) lds[(h * S1 + li) * RS + kz] = hold[h];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPP * m;
+            const int q = p <= N / 2 ? p : N - p;
+            T ar = v[m].x, br = v[m].y, ai = 0, bi = 0;
+            if (q == N / 2) {
+                ar = nyq[ra];
+                br = nyq[rb];
+            } else if (q > 0) { // c2r reads only the real part of DC and Nyquist
+                ai = lds[ra * RS + q];
+                bi = lds[rb * RS + q];
+            }
+            // Z = Xa + i Xb on 0..N/2, conj(Xa) + i conj(Xb) above
+            v[m] = p <= N / 2 ? C2<T>{ar - bi, ai + br} : C2<T>{ar + bi, br - ai};
+        }
+        fft_pencil_split<T, N, TPP, +1, 1, true>(v, my, stw, t);
+    } else {
+        const T *X = (const T *)in + mb * in_bs;
+        const int ja = 16 * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
+        const T *xa = X + ((long)i * N + ja) * N, *xb = X + ((long)i * N + jb) * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = C2<T>{xa[t + TPP * m], xb[t + TPP * m]};
+        __syncthreads(); // twiddles
+    }
+    // The pointwise work sits in a basic block of its own (the condition always
+    // holds): scheduled in one region with the FFT code around it, it took
+    // twice the registers.
+    if (blockIdx.x < gridDim.x) {
+        const int ja = 16 * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
+        const long ga = ((long)i * N + ja) * N, gb = ((long)i * N + jb) * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPP * m;
+            v[m].x = op(v[m].x, ga + p, acc);
+            v[m].y = op(v[m].y, gb + p, acc);
+        }
+    }
+    if constexpr (OUT_SPEC) {
+        fft_pencil_split<T, N, TPP, -1, 1, true>(v, my, stw, t);
+        // separation of the two rows: needs Z[k] and Z[N - k] (another thread's)
+        constexpr int EH = E / 2 + 1;
+        C2<T> xa[EH], xb[EH];
+        {
+            T ax[EH], bx[EH];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].x;
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < EH; ++m) {
+                const int k = t + TPP * m;
+                ax[m] = k <= N / 2 ? my[k] : (T)0;
+                bx[m] = k <= N / 2 ? my[(N - k) % N] : (T)0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].y;
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < EH; ++m) {
+                const int k = t + TPP * m;
+                const T ay = k <= N / 2 ? my[k] : (T)0, by = k <= N / 2 ? my[(N - k) % N] : (T)0;
+                // A = Z[k], B = Z[N-k]: Xa = (A + conj B)/2, Xb = -i (A - conj B)/2
+                const T hx = (T)0.5;
+                xa[m] = C2<T>{hx * (ax[m] + bx[m]), hx * (ay - by)};
+                xb[m] = C2<T>{hx * (ay + by), hx * (bx[m] - ax[m])};
+            }
+        }
+        __syncthreads();
+        // rows -> columns through the row buffer: real parts, then imaginary parts
+#pragma unroll
+        for (int m = 0; m < EH; ++m) {
+            const int k = t + TPP * m;
+            if (k < N / 2) {
+                lds[ra * RS + k] = xa[m].x;
+                lds[rb * RS + k] = xb[m].x;
+            } else if (k == N / 2) {
+                nyq[ra] = xa[m].x;
+                nyq[rb] = xb[m].x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < DH; ++h) c[h].x = lds[(h * S1 + li) * RS + kz];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < EH; ++m) {
+            const int k = t + TPP * m;
+            if (k < N / 2) {
+                lds[ra * RS + k] = xa[m].y;
+                lds[rb * RS + k] = xb[m].y;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < DH; ++h) c[h].y = lds[(h * S1 + li) * RS + kz];
+        // A: 16-point DFT over h, then the twiddle; digit k1 goes to position h
+        C2<T> *Wo = (C2<T> *)out + mb * out_bs;
+        auto stage_a = [&](int lc) {
+            dft<16, -1>(c);
+#pragma unroll
+            for (int h = 1; h < DH; ++h) c[h] = cmul(c[h], stw[twpad(lc * h)]);
+        };
+        stage_a(l);
+#pragma unroll
+        for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, ko, l, ki)] = c[h];
+        if (tid < S1) { // kz = N/2 columns (real after the r2c)
+            const int ln = S1 * lo + tid;
+#pragma unroll
+            for (int h = 0; h < DH; ++h) c[h] = C2<T>{nyq[h * S1 + tid], (T)0};
+            stage_a(ln);
+#pragma unroll
+            for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, (N / 2) / KT, ln, (N / 2) % KT)] = c[h];
+        }
+    }
+    write_partials<Op::NRED, Op::MAXM>(acc, red, part);
+}
+
+// ----------------------------------------------------------------------
+// Pass 2: one (h, ko, kg): the columns (l, ka), ki = KB kg + ka, over all i.
+//   FWD: B (16-point DFTs over l) on W, then forward FFTs along i
+//   op:  the spectral operator at frequency (i, J = h + 16 l, kz); PRELOAD
+//        operators read their state at the U offset
+//   INV: inverse FFTs along i, then B^-1, into W
+// Without FWD the input is read from the U layout, without INV the output is
+// written to it.
+// ----------------------------------------------------------------------
+template <typename T, int N, bool FWD, bool INV, class SOp, class Pro = ProNone>
+__global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
+    k_sj_cols(const C2<T> *src, C2<T> *dst, const C2<T> *__restrict__ tw, SOp op0, double *__restrict__ part,
+              const Ctl *ctl, Pro pro0, long bs)
+{
+    using S = SJ<T, N>;
+    constexpr int TPP = S::TPP, E = S::E, KT = S::KT, KB = S::KB, NC = S::NC, DL = S::DL;
+    constexpr int LS = NC + KB; // LDS stride of i: the line-role accesses then cover all banks
+    __shared__ T lds[N * LS];
+    __shared__ C2<T> stw[twiddle_slots<N>()];
+    __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
+    const int mb = blockIdx.y;
+    if (ctl && ctl[mb].stop) return;
+    const SOp op = bind_member(op0, mb);
+    if constexpr (!Pro::NONE) {
+        const Pro pro = bind_member(pro0, mb);
+        if ((int)blockIdx.x < pro.nblk()) pro(red);
+    }
+    src += mb * bs;
+    if (dst) dst += mb * bs;
+    load_twiddles<T, N>(stw, tw);
+    int h, ko, kg;
+    S::cblock(blockIdx.x, h, ko, kg);
+    // line role: thread (ia, ka) holds the DL values l of (i = ia, ki = KB kg + ka)
+    const int ka = threadIdx.x % KB, ia = threadIdx.x / KB;
+    const int kia = KB * kg + ka;
+    const bool va = ko * KT + kia < S::NH;
+    const long lbase = S::wof(ia, h, ko, 0, kia);
+    // column role: thread (cb, tb) holds column cb = (l, ka) at i = tb + TPP m
+    const int cb = threadIdx.x % NC, tb = threadIdx.x / NC;
+    const int lb = cb / KB, kzb = ko * KT + KB * kg + cb % KB;
+    const bool vb = kzb < S::NH;
+    const long ub = S::uof(h, ko, kg, 0, 0);
+    C2<T> v[E];
+    static_assert(E == DL, "one register array serves both roles");
+    C2<T> pre[SOp::PRELOAD ? E : 1]; // state, loaded at the start: its latency overlaps the transforms
+    double acc[SOp::NRED > 0 ? SOp::NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (SOp::NRED > 0 ? SOp::NRED : 1); ++q) acc[q] = 0.0;
+    if constexpr (FWD) {
+#pragma unroll
+        for (int q = 0; q < DL; ++q) v[q] = va ? src[lbase + q * KT] : C2<T>{0, 0};
+        if constexpr (SOp::PRELOAD) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) pre[m] = vb ? op.load_pre(ub + (tb + TPP * m) * NC + cb) : C2<T>{0, 0};
+        }
+        dft<16, -1>(v);
+        // line role -> column role: real parts, then imaginary parts
+        T hold[E];
+#pragma unroll
+        for (int q = 0; q < DL; ++q) {
+            lds[ia * LS + q * KB + ka] = v[q].x;
+            hold[q] = v[q].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m].x = lds[(tb + TPP * m) * LS + cb];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DL; ++q) lds[ia * LS + q * KB + ka] = hold[q];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m].y = lds[(tb + TPP * m) * LS + cb];
+        fft_pencil_split<T, N, TPP, -1, LS>(v, lds + cb, stw, tb);
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = vb ? src[ub + (tb + TPP * m) * NC + cb] : C2<T>{0, 0};
+    }
+    // The update in basic blocks of 4 elements (the conditions always hold):
+    // scheduled in one region, with its table gathers hoisted, it took far
+    // more registers.
+    if (vb) {
+        const int J = h + 16 * lb;
+#pragma unroll
+        for (int m0 = 0; m0 < E; m0 += 4) {
+            if (blockIdx.x < gridDim.x + m0) {
+#pragma unroll
+                for (int m = m0; m < m0 + 4; ++m) {
+                    const int ii = tb + TPP * m;
+                    op(v[m], pre[SOp::PRELOAD ? m : 0], ub + ii * NC + cb, ii, J, kzb, acc);
+                }
+            }
+        }
+    }
+    if constexpr (INV) {
+        fft_pencil_split<T, N, TPP, +1, LS>(v, lds + cb, stw, tb);
+        T hold[E];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            lds[(tb + TPP * m) * LS + cb] = v[m].x;
+            hold[m] = v[m].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DL; ++q) v[q].x = lds[ia * LS + q * KB + ka];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) lds[(tb + TPP * m) * LS + cb] = hold[m];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DL; ++q) v[q].y = lds[ia * LS + q * KB + ka];
+        dft<16, +1>(v);
+        if (va)
+#pragma unroll
+            for (int q = 0; q < DL; ++q) dst[lbase + q * KT] = v[q];
+    } else {
+        if (vb)
+#pragma unroll
+            for (int m = 0; m < E; ++m) dst[ub + (tb + TPP * m) * NC + cb] = v[m];
+    }
+    write_partials<SOp::NRED, SOp::MAXM>(acc, red, part);
+}
+
+} // namespace spm

@@ -1,6 +1,7 @@
 // grid_impl.h -- GridIface for one (precision, N): the pass sequences of the
 // transforms and operators of the reference's path, launched on ctx.stream.
 #pragma once
+#include "gf_split.h"
 #include "spm_ctx.h"
 
 #include <algorithm>
@@ -56,6 +57,25 @@ struct GridImpl final : GridIface {
         hipLaunchKernelGGL((k_cols<T, N, TI, FWD, INV, STORE, SOp, ProNone>), dim3((unsigned)nb, (unsigned)nm),
                            dim3(GI::COL_THREADS), 0, c.stream, (const CT *)src, (CT *)dst, ncols, ncols, 0L, nh, nhp,
                            1, (const CT *)c.tw, op, part, ctl, ProNone{}, bs);
+    }
+
+    // two-pass gradient-flow step (N = 256): pass 1 over rows, pass 2 over columns
+    using SJT = SJ<T, N>;
+    template <bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
+    static void sj_rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl, Pro pro = Pro{},
+                        int nm = 1, long in_bs = 0, long out_bs = 0)
+    {
+        hipLaunchKernelGGL((k_sj_rows<T, N, IN_SPEC, OUT_SPEC, Op, Pro>),
+                           dim3((unsigned)SJT::R_BLOCKS, (unsigned)nm), dim3(SJT::R_THREADS), 0, c.stream, in, out,
+                           (const CT *)c.tw, op, part, ctl, pro, in_bs, out_bs);
+    }
+    template <bool FWD, bool INV, class SOp, class Pro = ProNone>
+    static void sj_cols(Ctx &c, const void *src, void *dst, SOp op, double *part, const Ctl *ctl, Pro pro = Pro{},
+                        int nm = 1, long bs = 0)
+    {
+        hipLaunchKernelGGL((k_sj_cols<T, N, FWD, INV, SOp, Pro>), dim3((unsigned)SJT::C_BLOCKS, (unsigned)nm),
+                           dim3(SJT::C_THREADS), 0, c.stream, (const CT *)src, (CT *)dst, (const CT *)c.tw, op, part,
+                           ctl, pro, bs);
     }
 
     // ---------------------------------------------------------- transforms
@@ -171,6 +191,16 @@ struct GridImpl final : GridIface {
     void gf_init_member(Ctx &c, int m, const void *phi, double Tau, double Gamma) override
     {
         CT *Um = (CT *)c.U + (long)m * c.C, *Wm = (CT *)c.W + (long)m * c.C;
+        if constexpr (N == 256) {
+            if (c.gf2) { // U and W in the two-pass layout; slot 2 holds the density partials of phi
+                using S = SJT;
+                sj_rows<false, true>(c, phi, Wm, OpNone{}, nullptr, c.ctl + m);
+                sj_cols<true, false>(c, Wm, Um, SOpScale<T>{1.0 / (double)c.R}, nullptr, c.ctl + m);
+                sj_rows<false, true>(c, phi, Wm, OpGradFlowRow<T>{Tau, Gamma, nullptr, nullptr, 0},
+                                     c.slot(2) + (long)m * S::R_BLOCKS, c.ctl + m);
+                return;
+            }
+        }
         r2c(c, phi, Um);
         rows<false, true>(c, phi, Wm, OpGradFlowRow<T>{Tau, Gamma, nullptr, nullptr, 0}, c.slot(2) + (long)m * c.nb_rows,
                           c.ctl + m);
@@ -180,6 +210,14 @@ struct GridImpl final : GridIface {
     // phi of member m from its state spectrum (its W is the scratch)
     void gf_read_member(Ctx &c, int m, void *phi) override
     {
+        if constexpr (N == 256) {
+            if (c.gf2) {
+                CT *Um = (CT *)c.U + (long)m * c.C, *Wm = (CT *)c.W + (long)m * c.C;
+                sj_cols<false, true>(c, Um, Wm, SOpNone{}, nullptr, nullptr);
+                sj_rows<true, false>(c, Wm, nullptr, OpStore<T>{(T *)phi}, nullptr, nullptr);
+                return;
+            }
+        }
         c2r_ws(c, (const CT *)c.U + (long)m * c.C, phi, (CT *)c.W + (long)m * c.C);
     }
 
@@ -194,6 +232,19 @@ struct GridImpl final : GridIface {
         const int nm = c.nmem;
         const long bs = c.C;
         const bool nt = 2.0 * (double)bs * sizeof(CT) * nm > 200e6; // W + U beyond the 256 MB last-level cache
+        if constexpr (N == 256) {
+            if (c.gf2) {
+                using S = SJT;
+                sj_cols<true, true>(c, c.W, c.W,
+                                    SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
+                                    c.slot(3), c.ctl, ProSJDensity{c.slot(2), S::R_BLOCKS, c.slot(4)}, nm, bs);
+                sj_rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr, c.prm, 0}, c.slot(2), c.ctl,
+                                    ProSJFinalize{c.slot(3), S::C_BLOCKS, c.slot(4), (double)c.R, c.ctl, c.hist,
+                                                  3 * c.hist_cap},
+                                    nm, bs, bs);
+                return;
+            }
+        }
         cols_i<true, true, true>(c, c.W, c.W,
                                  SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
                                  c.slot(3), c.ctl, -1, -1, nm, bs);

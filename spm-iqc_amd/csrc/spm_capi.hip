@@ -216,6 +216,10 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
         return spm_fail(SPM_EINVAL, "spm_create: precision must be SPM_FP64 or SPM_FP32");
     if (!(p->L > 0) || !(p->kscale > 0) || !std::isfinite(p->Tau) || !std::isfinite(p->Gamma) || !(p->dt > 0))
         return spm_fail(SPM_EINVAL, "spm_create: L, kscale, dt must be > 0 and Tau, Gamma finite");
+    if (p->step_passes != 0 && p->step_passes != 2 && p->step_passes != 4)
+        return spm_fail(SPM_EINVAL, "spm_create: step_passes must be 0, 2 or 4");
+    if (p->step_passes == 2 && p->N != 256)
+        return spm_fail(SPM_ENOTSUP, "spm_create: the two-pass gradient-flow step exists for N = 256 only");
     spm_ctx *h = new spm_ctx();
     Ctx &c = h->c;
     c.N = p->N;
@@ -246,6 +250,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
                                         std::to_string(c.N));
     }
     c.shape = shape;
+    c.gf2 = p->step_passes == 2 || (p->step_passes == 0 && c.N == 256);
     c.nmem = nmem;
     c.g = make_grid(c.N, c.prec, shape);
     if (!c.g) {
@@ -560,7 +565,7 @@ int spm_gradflow(spm_ctx *h, void *phi, int max_iter, double res_tol, double gn_
             HIPCHK(hipStreamSynchronize(c.stream));
             if (((Ctl *)h->h_pin)->stop) break;
         }
-        c.g->c2r(c, c.U, phi);
+        c.g->gf_read_member(c, 0, phi);
     }
     HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -595,7 +600,7 @@ int spm_gf_steps(spm_ctx *h, int nsteps)
 int spm_gf_end(spm_ctx *h, void *phi, double *hist, int *iters)
 {
     CHECK_CTX(h);
-    if (phi) c.g->c2r(c, c.U, phi);
+    if (phi) c.g->gf_read_member(c, 0, phi);
     HIPCHK(hipMemcpyAsync(h->h_pin, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     const int it = ((Ctl *)h->h_pin)->iter;

@@ -54,6 +54,7 @@ struct Ctx {
     int nmem = 1;
     double *prm = nullptr;
     int shape = 0; // FFT pass shape (index into grid_shapes(N))
+    int gf2 = 0;   // N = 256: gradient-flow step in two passes (U and W in the two-pass layout)
 
     double *slot(int s) { return part + (long)s * part_slot; }
 };

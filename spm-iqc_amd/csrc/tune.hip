@@ -104,6 +104,9 @@ int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
 {
     spm_params q = *p;
     q.shape = v;
+    // the shape sets the passes of the four-pass step, which are the ones the
+    // other operators (LOBPCG, SPM forces) run; the two-pass step has its own
+    q.step_passes = 4;
     spm_ctx *h = nullptr;
     RCHK(spm_ctx_create(&q, nmem, 2L * steps + 2, &h));
     Ctx &c = h->c;

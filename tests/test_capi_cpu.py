@@ -53,3 +53,16 @@ def test_seeds(seeds):
     assert np.all(np.abs(np.linalg.norm(n, axis=1) - 26) < 1.0)
     x = seeds.gaussian(16, 0.05, 0)
     assert abs(x.mean()) < 1e-17 and abs(x.std() - 0.05) < 0.005
+
+
+def test_step_passes_validated_before_any_device_work(spm):
+    """spm_create rejects a bad step_passes value (and the two-pass step below
+    N = 256) during parameter checks, before it touches a device."""
+    import ctypes
+
+    lib = spm.lib()
+    h = ctypes.c_void_p()
+    for N, passes, want in ((64, 3, -1), (64, 2, -4), (128, 2, -4), (256, -1, -1)):
+        p = spm._params(N, spm.FP64, 26.0, 1.0, -0.0061, 0.30, 0.3, 0, passes)
+        assert lib.spm_create(ctypes.byref(p), ctypes.byref(h)) == want
+        assert not h.value

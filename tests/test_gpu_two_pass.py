@@ -1,0 +1,103 @@
+"""The two-pass gradient-flow step of a 256^3 grid (the j transform split
+between the row pass and the i pass; the default at N = 256) against the
+oracle, against the four-pass step, in batches, and through its own
+state layout.  Tolerances as for every get_min_u_F check: FP64 energy 1e-12
+relative; FP32 against FP64 1e-6 relative."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+
+    assert _t.cuda.is_available()
+    return _t
+
+
+def _dev_seed(seeds, torch, kind, s, dtype, kscale=1.0):
+    return seeds.device_field(torch, N, kind, 26.0, kscale, 0.1, s, dtype=dtype)
+
+
+def test_two_pass_gradflow_matches_oracle(spm, po, seeds, torch):
+    """get_min_u_F (LPsrc.cpp:40-117) at 256^3, FP64: per-step E, ||grad F||,
+    residual and the final field against the oracle."""
+    print("[case] two-pass gradflow 256 vs oracle", flush=True)
+    steps = 4
+    phi = seeds.icosahedral_nucleus(N, seed=0)
+    o = po.Oracle(N)
+    uR = phi.astype(np.complex128)
+    uF = o.my_fft(uR)
+    it, E, res, gn = o.get_min_u_F(uR, uF, steps, 1e-16, gnorm=True)
+    with spm.Grid(N, 0) as g:
+        pt = torch.as_tensor(phi).cuda()
+        itg, hist = g.gradflow(pt, steps, 1e-16, 0.0, check_every=2)
+        out = pt.cpu().numpy()
+    assert itg == it == steps
+    assert np.all(np.abs(hist[:, 0] - E) <= 1e-12 * np.abs(E))
+    assert np.all(np.abs(hist[:, 1] - gn) <= 1e-9 * gn + 1e-14)
+    assert np.all(np.abs(hist[:, 2] - res) <= 1e-8 * res + 1e-15)
+    assert np.abs(out - uR.real).max() <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_two_pass_equals_four_pass(spm, seeds, torch, prec):
+    """Same iterations as the axis-by-axis step: FP64 to rounding (1e-13),
+    FP32 to its own rounding (1e-6 relative energy, 1e-5 field)."""
+    print(f"[case] two vs four passes prec={prec}", flush=True)
+    steps = 40
+    out = {}
+    for passes in (2, 4):
+        with spm.Grid(N, prec, kscale=2 * np.pi, Tau=0.005, step_passes=passes) as g:
+            t = _dev_seed(seeds, torch, "bcc", 3, g.dtype, 2 * np.pi)
+            it, hist = g.gradflow(t, steps, -1.0, 0.0, check_every=16)
+            out[passes] = (it, hist.copy(), t.double().cpu().numpy())
+    (i2, h2, f2), (i4, h4, f4) = out[2], out[4]
+    assert i2 == i4 == steps
+    tol_e, tol_f = (1e-13, 1e-13) if prec == 0 else (1e-6, 1e-5)
+    assert np.all(np.abs(h2[:, 0] - h4[:, 0]) <= tol_e * np.abs(h4[:, 0]))
+    assert np.all(np.abs(h2[:, 1] - h4[:, 1]) <= tol_e * 100 * h4[:, 1])
+    assert np.abs(f2 - f4).max() <= tol_f * np.abs(f4).max()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_two_pass_layout_round_trip(spm, seeds, torch, prec):
+    """gf_begin then gf_end with no step: phi -> (row pass, i pass) -> state
+    -> (i pass, row pass) -> phi, through the two-pass state layout."""
+    print(f"[case] two-pass round trip prec={prec}", flush=True)
+    with spm.Grid(N, prec) as g:
+        phi = _dev_seed(seeds, torch, "nucleus", 1, g.dtype)
+        g.gf_begin(phi, 10, 1e-16, 0.0)
+        back = torch.empty_like(phi)
+        it, _ = g.gf_end(back, 10)
+    assert it == 0
+    err = (back - phi).abs().max().item() / phi.abs().max().item()
+    assert err < (64 * 2e-15 if prec == 0 else 3e-6)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_two_pass_batch_equals_single_runs(spm, seeds, torch, prec):
+    """Batch members (own Tau, Gamma, limits) give exactly the single runs."""
+    print(f"[case] two-pass batch prec={prec}", flush=True)
+    members = [("bcc", -0.02, 0.20, 21), ("fcc", 0.01, 0.40, 30)]
+    kw = dict(kscale=2 * np.pi)
+    with spm.Batch(N, prec, capacity=2, hist_cap=64, **kw) as b:
+        phis = [_dev_seed(seeds, torch, k, i, b.dtype, 2 * np.pi) for i, (k, *_) in enumerate(members)]
+        for m, ((_, Tau, Gamma, mi), t) in enumerate(zip(members, phis)):
+            b.load(m, t, Tau, Gamma, mi, 1e-16, 0.0)
+        b.steps(32)
+        running, it, _ = b.poll()
+        assert running == 0 and list(it[:2]) == [21, 30]
+        got = [b.read(m) for m in range(2)]
+    for m, (_, Tau, Gamma, mi) in enumerate(members):
+        with spm.Grid(N, prec, Tau=Tau, Gamma=Gamma, **kw) as g:
+            t = phis[m].clone()
+            it1, h1 = g.gradflow(t, mi, 1e-16, 0.0, check_every=7)
+        phi_b, h_b = got[m]
+        assert it1 == mi and len(h_b) == mi
+        assert np.array_equal(h_b, h1), "history of member %d differs from the single run" % m
+        assert torch.equal(phi_b, t), "field of member %d differs" % m
