@@ -37,16 +37,20 @@ def algorithmic_bytes_per_step(N: int, s: int) -> int:
     return 2 * R * s + 5 * C * 2 * s
 
 
-STEP_KERNELS = ("SOpGradFlowStep", "ProGradFlowReduce", "ProGradFlowFinalize", "true, false, true, SOpNone")
+# the kernels of one step: two passes (N = 256) or four
+STEP_KERNELS_2 = (("k_sj_cols", "SOpGradFlowStep"), ("k_sj_rows", "ProSJFinalize"))
+STEP_KERNELS_4 = (("k_cols", "SOpGradFlowStep"), ("k_cols", "ProGradFlowReduce"), ("k_rows", "ProGradFlowFinalize"),
+                  ("k_cols", "true, false, true, SOpNone, ProNone"))
 
 
-def measured_traffic(N, dtype):
+def measured_traffic(N, dtype, passes):
     """HBM bytes of one step, summed over its kernels, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/gradflow_step_<N>.md);
-    None if there is none for this size."""
+    None if there is none for this size and step."""
     path = os.path.join(ROOT, "profiles", "gradflow_step_%d.md" % N)
     if not os.path.exists(path):
         return None
+    kernels = STEP_KERNELS_2 if passes == 2 else STEP_KERNELS_4
     total, seen = 0.0, set()
     for line in open(path):
         cells = [c.strip() for c in line.split("|")]
@@ -55,11 +59,11 @@ def measured_traffic(N, dtype):
         name = cells[1]
         if ("<%s, %d," % (dtype, N)) not in name:
             continue
-        for k in STEP_KERNELS:
-            if k in name and k not in seen and ("ProNone" in name or k != "true, false, true, SOpNone"):
+        for k in kernels:
+            if name.startswith("`" + k[0] + "<") and k[1] in name and k not in seen:
                 seen.add(k)
                 total += (float(cells[4]) + float(cells[5])) * 1e6
-    return total if len(seen) == len(STEP_KERNELS) else None
+    return total if len(seen) == len(kernels) else None
 
 
 def seed_field(seeds, N):
@@ -67,8 +71,8 @@ def seed_field(seeds, N):
     return seeds.icosahedral_nucleus(N, L=26.0, kscale=1.0, amp=0.1, seed=0)
 
 
-def time_gpu(spm, torch, N, prec, steps, warmup, phi0, shape=0):
-    g = spm.Grid(N, prec, shape=shape)
+def time_gpu(spm, torch, N, prec, steps, warmup, phi0, shape=0, passes=0):
+    g = spm.Grid(N, prec, shape=shape, step_passes=passes)
     phi = torch.as_tensor(phi0).to(g.dtype).cuda()
     total = warmup + steps
     g.gf_begin(phi, total + 1, res_tol=-1.0, gn_tol=0.0)  # no early stop inside the timed region
@@ -139,7 +143,9 @@ def main():
                     help="also run one plain 512 MiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--shape", default="-1",
                     help="FFT pass shape index (threads per pencil of the three passes), or FP64,FP32 indices; "
-                         "-1: measure them all first and use the fastest (the tuner)")
+                         "-1: measure them all first and use the fastest (the tuner; only for the four-pass step)")
+    ap.add_argument("--step-passes", type=int, default=0, choices=(0, 2, 4),
+                    help="array passes per step: 0 = the library default (2 at N = 256, else 4)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests of the serving stream (p50/p99 latency); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=12.0)
@@ -163,17 +169,18 @@ def main():
         torch.cuda.synchronize()
         del a, b
     shapes = spm.shapes(N)
+    passes = args.step_passes or (2 if N == 256 else 4)
     tuned = {}
     want = [int(x) for x in str(args.shape).split(",")]
     want = want * 2 if len(want) == 1 else want
     for prec in (spm.FP64, spm.FP32):
-        if want[prec] >= 0:
-            tuned[prec] = (want[prec], None)
-        else:  # the tuner: every shape timed on 30 graph-replayed steps of a zero field
+        if want[prec] >= 0 or passes == 2:  # the two-pass step has no shape to tune
+            tuned[prec] = (max(want[prec], 0), None)
+        else:  # the tuner: every shape timed on 30 graph-replayed steps of a noise field
             best, us = spm.tune(N, prec, nmem=1, steps=30)
             tuned[prec] = (best, us)
-    r64 = time_gpu(spm, torch, N, spm.FP64, args.steps, args.warmup, phi0, tuned[spm.FP64][0])
-    r32 = time_gpu(spm, torch, N, spm.FP32, args.steps, args.warmup, phi0, tuned[spm.FP32][0])
+    r64 = time_gpu(spm, torch, N, spm.FP64, args.steps, args.warmup, phi0, tuned[spm.FP64][0], passes)
+    r32 = time_gpu(spm, torch, N, spm.FP32, args.steps, args.warmup, phi0, tuned[spm.FP32][0], passes)
 
     sps64 = args.steps / (r64["event_ms"] / 1e3)
     sps32 = args.steps / (r32["event_ms"] / 1e3)
@@ -208,15 +215,20 @@ def main():
         "data": "synthetic (seeded icosahedral nucleus, 30-vector star |q|=1, sphere r=6, tanh edge 1)",
         "config": {"workload": "semi-implicit gradient flow (get_min_u_F), one %d^3 field, L=26, k=n/L, "
                                "Tau=-0.0061, Gamma=0.30, dt=0.3" % N, "N": N, "batch": 1,
-                   "fft_shape_fp64": shapes[tuned[spm.FP64][0]], "fft_shape_fp32": shapes[tuned[spm.FP32][0]]},
+                   "step_passes": passes,
+                   "fft_shape_fp64": shapes[tuned[spm.FP64][0]] if passes == 4 else None,
+                   "fft_shape_fp32": shapes[tuned[spm.FP32][0]] if passes == 4 else None},
         "mixed": {"dtype": "f32 fields/FFTs, f64 sums", "value": round(sps32, 3),
                   "ms_per_step": round(r32["event_ms"] / args.steps, 4),
                   "speedup_vs_fp64": round(sps32 / sps64, 3),
                   "roofline_frac": round(ach32 / HBM_PEAK, 4)},
-        "roofline": {"bound": "hbm", "kernel": "gradient-flow step (4 launches: i-axis fwd FFT + update + inv FFT; "
-                                               "j-axis inv FFT; k-axis c2r + f(phi) + r2c; j-axis fwd FFT)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("gradient-flow step (2 launches: j stage B + i-axis FFT + update + inverse + B^-1; "
+                                "j stage A^-1 + k-axis c2r + f(phi) + r2c + A)" if passes == 2 else
+                                "gradient-flow step (4 launches: i-axis fwd FFT + update + inv FFT; "
+                                "j-axis inv FFT; k-axis c2r + f(phi) + r2c; j-axis fwd FFT)"),
                      "achieved": round(ach64 / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(ach64 / HBM_PEAK, 4), "traffic": measured_traffic(N, "double"),
+                     "frac": round(ach64 / HBM_PEAK, 4), "traffic": measured_traffic(N, "double", passes),
                      "algorithmic_bytes_per_step": B64},
         "cpu_baseline": cpu,
         "serving": serving,
