@@ -160,8 +160,10 @@ struct ProSJFinalize {
 // of a thread (the 16 h of one (l, kz < N/2)) does stage A; the row role
 // (TPP threads per pair) does the row FFTs; the row buffer in LDS converts.
 // ----------------------------------------------------------------------
+// At most 168 registers (3 waves per SIMD): FP64 needs 194 otherwise; the two it
+// then spills cost less than the third block per CU gains (2 % of the step).
 template <typename T, int N, bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
-__global__ void __launch_bounds__((SJ<T, N>::R_THREADS))
+__global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     k_sj_rows(const void *in, void *out, const C2<T> *__restrict__ tw, Op op0, double *__restrict__ part,
               const Ctl *ctl, Pro pro0, long in_bs, long out_bs)
 {
@@ -187,7 +189,12 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS))
     S::rblock(blockIdx.x, i, lo);
     const int tid = threadIdx.x;
     // column role: the 16 h values of (l = S1 lo + li, kz)
-    const int ki = tid % KT, li = (tid / KT) % S1, ko = tid / (KT * S1);
+    // FP64: a wave spans both l of 32 kz; FP32: one l and 64 kz, so that its row-buffer
+    // accesses of 4-byte values fall in distinct banks (3.5 % of the FP32 step)
+    constexpr bool LMAJ = sizeof(T) == 4;
+    const int li = LMAJ ? tid / (N / 2) : (tid / KT) % S1;
+    const int kzc = LMAJ ? tid % (N / 2) : (tid / (KT * S1)) * KT + tid % KT;
+    const int ki = kzc % KT, ko = kzc / KT;
     const int kz = ko * KT + ki, l = S1 * lo + li;
     // row role: pair pp = rows ra, ra + 1 (row r = h S1 + li)
     const int t = tid % TPP, pp = tid / TPP;
