@@ -78,6 +78,21 @@ struct GridImpl final : GridIface {
                            ctl, pro, bs);
     }
 
+    // A transform round trip r2c -> spectral op -> c2r as pass 1, pass 2, pass 1
+    // (N = 256 in two-pass mode): six array passes instead of ten.  opin runs on
+    // the input field (its sums into pin), opout on the output (sums into pout);
+    // mid (host) runs between the first pass and the second (scalar reductions).
+    template <class OpIn, class SOp, class OpOut, class Mid>
+    static void sj_round_trip(Ctx &c, const void *in, OpIn opin, double *pin, Mid mid, SOp sop, OpOut opout,
+                              double *pout)
+    {
+        sj_rows<false, true>(c, in, c.W, opin, pin, nullptr);
+        mid();
+        sj_cols<true, true>(c, c.W, c.W, sop, nullptr, nullptr);
+        sj_rows<true, false>(c, c.W, nullptr, opout, pout, nullptr);
+    }
+    static bool two_pass(const Ctx &c) { return N == 256 && c.gf2; }
+
     // ---------------------------------------------------------- transforms
     void r2c(Ctx &c, const void *x, void *S) override
     {
@@ -128,6 +143,17 @@ struct GridImpl final : GridIface {
     // ngrad_cam (LPsrc.cpp:119-173)
     void ngrad(Ctx &c, const void *phi, void *grad) override
     {
+        if constexpr (N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(
+                    c, phi, OpNgradIn<T>{c.Tau, c.Gamma}, c.slot(0),
+                    [&] { launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_F); },
+                    SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
+                    OpNgradOut<T>{c.Tau, c.Gamma, (const T *)phi, c.scal + SC_MEAN_F, (T *)grad}, c.slot(1));
+                launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_GN2);
+                return;
+            }
+        }
         rows<false, true>(c, phi, c.W, OpNgradIn<T>{c.Tau, c.Gamma}, c.slot(0));
         launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_F);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
@@ -141,6 +167,16 @@ struct GridImpl final : GridIface {
     // hv_cam (LOBPCG.cpp:113-166)
     void hv(Ctx &c, const void *phi, const void *v, void *hvo) override
     {
+        if constexpr (N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(
+                    c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0),
+                    [&] { launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV); },
+                    SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
+                    OpHvOut<T>{c.Tau, c.Gamma, (const T *)phi, (const T *)v, c.scal + SC_MEAN_CV, (T *)hvo}, nullptr);
+                return;
+            }
+        }
         rows<false, true>(c, v, c.W, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0));
         launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
@@ -155,6 +191,13 @@ struct GridImpl final : GridIface {
     // minus its mean (= the same with the DC set to zero).
     void precond(Ctx &c, void *w) override
     {
+        if constexpr (N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(c, w, OpNone{}, nullptr, [] {}, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab},
+                              OpStore<T>{(T *)w}, nullptr);
+                return;
+            }
+        }
         rows<false, true>(c, w, c.W, OpNone{}, nullptr);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
         cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab}, nullptr);
@@ -164,6 +207,13 @@ struct GridImpl final : GridIface {
 
     void gconv(Ctx &c, const void *in, void *out) override
     {
+        if constexpr (N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(c, in, OpNone{}, nullptr, [] {}, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
+                              OpStore<T>{(T *)out}, nullptr);
+                return;
+            }
+        }
         rows<false, true>(c, in, c.W, OpNone{}, nullptr);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
         cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, nullptr);
@@ -173,6 +223,13 @@ struct GridImpl final : GridIface {
 
     void force(Ctx &c, void *x, const void *v, const double *s, double coef) override
     {
+        if constexpr (N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(c, x, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr, [] {},
+                              SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, OpStore<T>{(T *)x}, nullptr);
+                return;
+            }
+        }
         rows<false, true>(c, x, c.W, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
         cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, nullptr);

@@ -1,7 +1,7 @@
 """Fixed 256^3 workloads of the path, for profilers and quick timing: one
 command per kernel family, printing its own timing.
 
-    python spm-iqc_amd/workloads.py gradflow [--precision fp64|fp32] [--steps 50] [--shape -1]
+    python spm-iqc_amd/workloads.py gradflow [--precision fp64|fp32] [--steps 50] [--shape -1] [--step-passes 0]
     python spm-iqc_amd/workloads.py lobpcg   [--precision ...] [--iters 5]
     python spm-iqc_amd/workloads.py spm      [--precision ...] [--iters 20]
 
@@ -35,10 +35,10 @@ def _shape(N, prec, shape):
     return best
 
 
-def gradflow(N, prec, steps, shape):
+def gradflow(N, prec, steps, shape, passes=0):
     torch = spm._torch()
     shape = _shape(N, prec, shape)
-    with spm.Grid(N, prec, shape=shape) as g:
+    with spm.Grid(N, prec, shape=shape, step_passes=passes) as g:
         phi = _field(torch, g, N)
         g.gf_begin(phi, steps + 33, res_tol=-1.0, gn_tol=0.0)
         g.gf_steps(32)
@@ -53,10 +53,10 @@ def gradflow(N, prec, steps, shape):
     print("gradflow N %d prec %d shape %s: %d steps, %.1f us/step" % (N, prec, spm.shapes(N)[shape], steps, us))
 
 
-def lobpcg(N, prec, iters, shape):
+def lobpcg(N, prec, iters, shape, passes=0):
     torch = spm._torch()
     shape = _shape(N, prec, shape)
-    with spm.Grid(N, prec, shape=shape) as g:
+    with spm.Grid(N, prec, shape=shape, step_passes=passes) as g:
         phi = _field(torch, g, N)
         gen = torch.Generator(device="cuda")
         gen.manual_seed(0)
@@ -72,10 +72,10 @@ def lobpcg(N, prec, iters, shape):
                                                                               ritz[:4]))
 
 
-def saddle(N, prec, iters, shape):
+def saddle(N, prec, iters, shape, passes=0):
     torch = spm._torch()
     shape = _shape(N, prec, shape)
-    with spm.Grid(N, prec, shape=shape) as g:
+    with spm.Grid(N, prec, shape=shape, step_passes=passes) as g:
         phi = _field(torch, g, N)
         gen = torch.Generator(device="cuda")
         gen.manual_seed(1)
@@ -98,17 +98,19 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--shape", default="-1", help="FFT pass shape index, or FP64,FP32 indices; -1: the tuner's choice")
+    ap.add_argument("--step-passes", type=int, default=0, choices=(0, 2, 4),
+                    help="array passes per step and transform round trip: 0 = the library default")
     a = ap.parse_args(argv)
     precs = {"fp64": [spm.FP64], "fp32": [spm.FP32], "both": [spm.FP64, spm.FP32]}[a.precision]
     want = [int(x) for x in a.shape.split(",")]
     want = want * 2 if len(want) == 1 else want
     for p in precs:
         if a.workload == "gradflow":
-            gradflow(a.N, p, a.steps, want[p])
+            gradflow(a.N, p, a.steps, want[p], a.step_passes)
         elif a.workload == "lobpcg":
-            lobpcg(a.N, p, a.iters or 5, want[p])
+            lobpcg(a.N, p, a.iters or 5, want[p], a.step_passes)
         else:
-            saddle(a.N, p, a.iters or 20, want[p])
+            saddle(a.N, p, a.iters or 20, want[p], a.step_passes)
 
 
 if __name__ == "__main__":

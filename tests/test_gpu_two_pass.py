@@ -101,3 +101,72 @@ def test_two_pass_batch_equals_single_runs(spm, seeds, torch, prec):
         assert it1 == mi and len(h_b) == mi
         assert np.array_equal(h_b, h1), "history of member %d differs from the single run" % m
         assert torch.equal(phi_b, t), "field of member %d differs" % m
+
+
+def _rel(a, b):
+    return (a - b).abs().max().item() / max(b.abs().max().item(), 1e-300)
+
+
+def test_two_pass_ngrad_matches_oracle(spm, po, seeds, torch):
+    """ngrad_cam (LPsrc.cpp:119-173) at 256^3 through the two-pass round trip,
+    FP64, against the oracle (1e-10 max norm, as at the other sizes)."""
+    print("[case] two-pass ngrad 256 vs oracle", flush=True)
+    phi = seeds.icosahedral_nucleus(N, seed=2)
+    o = po.Oracle(N)
+    g_ref = o.ngrad_cam(phi + 0j).real
+    with spm.Grid(N, 0) as g:
+        gr, nrm = g.ngrad(torch.as_tensor(phi).cuda())
+        gr = gr.cpu().numpy()
+    assert np.abs(gr - g_ref).max() <= 1e-10
+    assert abs(nrm - o.nrmp(g_ref + 0j)) <= 1e-10 * nrm
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_two_pass_operators_equal_four_pass(spm, seeds, torch, prec):
+    """hv_cam, precond_camnew, ngrad_cam (LOBPCG.cpp:113-199, LPsrc.cpp:119-173)
+    and the SPM forces (SPM.cpp:34-168) through the two-pass round trip equal
+    the axis-by-axis passes to rounding (FP64 1e-13, FP32 2e-6 relative)."""
+    print(f"[case] two-pass operators prec={prec}", flush=True)
+    tol = 1e-13 if prec == 0 else 2e-6
+    out = {}
+    for passes in (2, 4):
+        with spm.Grid(N, prec, step_passes=passes) as g:
+            phi = _dev_seed(seeds, torch, "nucleus", 4, g.dtype)
+            gen = torch.Generator(device="cuda").manual_seed(5)
+            v = (torch.rand((N, N, N), generator=gen, dtype=torch.float64, device="cuda") - 0.5).to(g.dtype)
+            r = {"hv": g.hv(phi, v).clone(), "precond": g.precond(v.clone())}
+            gr, nrm = g.ngrad(phi)
+            r["ngrad"], r["nrm"] = gr.clone(), nrm
+            x = phi + 0.01 * v
+            r["natural"] = g.natural_force(x.clone(), v / g.nrmp(v)).clone()
+            r["minmax"] = g.minmax_force(x.clone(), v / g.nrmp(v)).clone()
+            out[passes] = r
+    for k in ("hv", "precond", "ngrad", "natural", "minmax"):
+        assert _rel(out[2][k].double(), out[4][k].double()) < tol, k
+    assert abs(out[2]["nrm"] - out[4]["nrm"]) <= tol * out[4]["nrm"]
+
+
+def test_two_pass_lobpcg_and_spm_equal_four_pass(spm, seeds, torch):
+    """hiosd_lobpcg_inip and Spring_Pairing at 256^3 over the two-pass round
+    trips follow the axis-by-axis runs (Ritz values 1e-9, SPM residual 1e-9
+    relative after 6 iterations)."""
+    print("[case] two-pass lobpcg/spm", flush=True)
+    out = {}
+    for passes in (2, 4):
+        with spm.Grid(N, 0, step_passes=passes) as g:
+            phi = _dev_seed(seeds, torch, "nucleus", 0, g.dtype)
+            gen = torch.Generator(device="cuda").manual_seed(0)
+            start = torch.rand((4, N, N, N), generator=gen, dtype=torch.float64, device="cuda") * 2 - 1
+            V = g.lobpcg_init(start)
+            del start
+            ritz = g.lobpcg(phi, V, 4)
+            del V
+            v = (torch.rand((N, N, N), generator=gen, dtype=torch.float64, device="cuda") - 0.5)
+            v /= g.nrmp(v)
+            p0, p1 = phi - 0.01 * v, phi + 0.01 * v
+            it, res, _ = g.spring_pairing(p0, p1, 6, 0.0)
+            out[passes] = (ritz.copy(), it, res, p0.cpu())
+    (r2, i2, s2, f2), (r4, i4, s4, f4) = out[2], out[4]
+    assert np.all(np.abs(r2 - r4) <= 1e-9)
+    assert i2 == i4 == 6 and abs(s2 - s4) <= 1e-9 * s4
+    assert (f2 - f4).abs().max().item() <= 1e-12
