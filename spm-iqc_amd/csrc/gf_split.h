@@ -1,20 +1,21 @@
-// gf_split.h -- the gradient-flow step of a 256^3 grid in two passes over the
-// work array instead of four (gfx950, HIP).
+// gf_split.h -- the gradient-flow step of a 128^3 or 256^3 grid in two passes
+// over the work array instead of four (gfx950, HIP).
 //
-// The length-256 transform along j is split by one Cooley-Tukey step,
-// j = 16 h + l:
-//   stage A: 16-point DFTs over h for each l (rows 16 apart in one i plane),
-//            then the twiddle exp(-2 pi i l k1 / 256); digit k1 is stored at h;
+// The length-N transform along j is split by one Cooley-Tukey step,
+// j = 16 h + l (h < DH = N/16, l < 16):
+//   stage A: DH-point DFTs over h for each l (rows 16 apart in one i plane),
+//            then the twiddle exp(-2 pi i l k1 / N); digit k1 is stored at h;
 //   stage B: 16-point DFTs over l for each k1 (16 adjacent rows); digit k2 is
-//            stored at l, so position (h, l) holds frequency J = h + 16 l.
+//            stored at l, so position (h, l) holds frequency J = h + DH l.
 // Stage A rides in the row (kz) pass and stage B in the i pass, so one
 // iteration of get_min_u_F (LPsrc.cpp:52-111) is
 //   pass 1 (rows):    A^-1, c2r, f(phi) and the density sum, r2c, A     W in/out
 //   pass 2 (columns): B, i FFT, update with U and its sums, i FFT^-1, B^-1
 //                                                          W in/out, U in/out
 // six array passes per step instead of the ten of the axis-by-axis passes.
+// The same passes make the transform round trips of the other operators.
 //
-// W and U of one field are stored as [i][h][ko][l][ki], kz = KT ko + ki, with
+// W of one field is stored as [i][h][ko][l][ki], kz = KT ko + ki, with
 // KT = 2 (FP64) or 4 (FP32).  A pass-1 block reads, per (h, ko), the 64-byte
 // half of a 128-byte line (2 l x KT ki); a pass-2 block reads, per i, every
 // other 16 bytes of 512 (16 l x KT/2 ki).  The two blocks sharing those lines
@@ -26,9 +27,9 @@
 namespace spm {
 
 template <typename T, int N> struct SJ {
-    static_assert(N == 256, "two-pass gradient-flow step: N = 256");
+    static_assert(N == 128 || N == 256, "two-pass gradient-flow step: N = 128 or 256");
     static constexpr int NH = N / 2 + 1;                // valid kz
-    static constexpr int DH = 16, DL = N / DH;          // ranges of the digits h and l
+    static constexpr int DL = 16, DH = N / DL;          // ranges of the digits h and l: j = DL h + l
     static constexpr int KT = sizeof(T) == 8 ? 2 : 4;   // kz per contiguous group
     static constexpr int KO = (NH + KT - 1) / KT;       // kz groups
     static constexpr long HSTR = (long)KO * DL * KT;    // stride of h
@@ -38,12 +39,11 @@ template <typename T, int N> struct SJ {
     {
         return (long)i * ISTR + (long)h * HSTR + ((long)ko * DL + l) * KT + ki;
     }
-    static constexpr int TPP = 16, E = N / TPP;
-    // pass 1: S1 = 2 l of one i (16 S1 rows, 8 S1 pairs of TPP threads); block b
-    // and b + 8 take l pairs 2 lp and 2 lp + 1 (one line)
-    static constexpr int S1 = 2;
+    // pass 1: S1 = 2 l of one i (DH S1 rows, DH S1 / 2 pairs of RTPP threads);
+    // block b and b + 8 take l pairs 2 lp and 2 lp + 1 (one line)
+    static constexpr int S1 = 2, RTPP = 16;
     static constexpr int NP = DH * S1 / 2;
-    static constexpr int R_THREADS = NP * TPP;
+    static constexpr int R_THREADS = NP * RTPP;
     static constexpr long R_BLOCKS = (long)N * (DL / S1);
     __device__ static void rblock(int b, int &i, int &lo)
     {
@@ -54,9 +54,12 @@ template <typename T, int N> struct SJ {
     }
     // pass 2: (h, ko, half kg of the ki): KB = KT/2 ki, NC = DL KB columns (l, ka);
     // block b and b + 8 take kg = 0 and 1 of one (h, ko)
+    // CTPP threads per i pencil, DL values each (one register array serves the
+    // line role, DL values of l, and the column role, DL values of i)
     static constexpr int KB = KT / 2;
     static constexpr int NC = DL * KB;
-    static constexpr int C_THREADS = NC * TPP;
+    static constexpr int CTPP = N / DL;
+    static constexpr int C_THREADS = NC * CTPP;
     static constexpr long C_BLOCKS = 2L * DH * KO;
     __device__ static void cblock(int b, int &h, int &ko, int &kg)
     {
@@ -168,7 +171,8 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
               const Ctl *ctl, Pro pro0, long in_bs, long out_bs)
 {
     using S = SJ<T, N>;
-    constexpr int TPP = S::TPP, E = S::E, NP = S::NP, S1 = S::S1, DH = S::DH, KT = S::KT, NR = 2 * NP;
+    constexpr int TPP = S::RTPP, E = N / TPP, NP = S::NP, S1 = S::S1, DH = S::DH, DL = S::DL, KT = S::KT;
+    constexpr int NR = 2 * NP;
     constexpr int RS = N / 2 + 8;                      // row stride of the row buffer (reals)
     constexpr int RSTR = pencil_slots<N, true>() + 1;  // pencil exchange stride (reals)
     constexpr int NL = NR * RS > NP * RSTR ? NR * RS : NP * RSTR;
@@ -220,7 +224,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
             w.y = -w.y;
             c[h] = cmul(c[h], w);
         }
-        dft<16, +1>(c);
+        dft<DH, +1>(c);
         // to the row buffer: real parts, then imaginary parts
         T hold[DH];
 #pragma unroll
@@ -241,7 +245,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
                     c[h] = cmul(c[h], w);
                 }
             }
-            dft<16, +1>(c);
+            dft<DH, +1>(c);
 #pragma unroll
             for (int h = 0; h < DH; ++h) nyq[h * S1 + tid] = c[h].x;
         }
@@ -274,7 +278,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         fft_pencil_split<T, N, TPP, +1, 1, true>(v, my, stw, t);
     } else {
         const T *X = (const T *)in + mb * in_bs;
-        const int ja = 16 * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
+        const int ja = DL * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
         const T *xa = X + ((long)i * N + ja) * N, *xb = X + ((long)i * N + jb) * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = C2<T>{xa[t + TPP * m], xb[t + TPP * m]};
@@ -284,7 +288,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     // holds): scheduled in one region with the FFT code around it, it took
     // twice the registers.
     if (blockIdx.x < gridDim.x) {
-        const int ja = 16 * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
+        const int ja = DL * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
         const long ga = ((long)i * N + ja) * N, gb = ((long)i * N + jb) * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -355,7 +359,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         // A: 16-point DFT over h, then the twiddle; digit k1 goes to position h
         C2<T> *Wo = (C2<T> *)out + mb * out_bs;
         auto stage_a = [&](int lc) {
-            dft<16, -1>(c);
+            dft<DH, -1>(c);
 #pragma unroll
             for (int h = 1; h < DH; ++h) c[h] = cmul(c[h], stw[twpad(lc * h)]);
         };
@@ -377,7 +381,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
 // ----------------------------------------------------------------------
 // Pass 2: one (h, ko, kg): the columns (l, ka), ki = KB kg + ka, over all i.
 //   FWD: B (16-point DFTs over l) on W, then forward FFTs along i
-//   op:  the spectral operator at frequency (i, J = h + 16 l, kz); PRELOAD
+//   op:  the spectral operator at frequency (i, J = h + DH l, kz); PRELOAD
 //        operators read their state at the U offset
 //   INV: inverse FFTs along i, then B^-1, into W
 // Without FWD the input is read from the U layout, without INV the output is
@@ -389,7 +393,7 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
               const Ctl *ctl, Pro pro0, long bs)
 {
     using S = SJ<T, N>;
-    constexpr int TPP = S::TPP, E = S::E, KT = S::KT, KB = S::KB, NC = S::NC, DL = S::DL;
+    constexpr int TPP = S::CTPP, E = N / TPP, KT = S::KT, KB = S::KB, NC = S::NC, DL = S::DL, DH = S::DH;
     constexpr int LS = NC + KB; // LDS stride of i: the line-role accesses then cover all banks
     __shared__ T lds[N * LS];
     __shared__ C2<T> stw[twiddle_slots<N>()];
@@ -455,7 +459,7 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
     // scheduled in one region, with its table gathers hoisted, it took far
     // more registers.
     if (vb) {
-        const int J = h + 16 * lb;
+        const int J = h + DH * lb;
 #pragma unroll
         for (int m0 = 0; m0 < E; m0 += 4) {
             if (blockIdx.x < gridDim.x + m0) {

@@ -115,7 +115,9 @@ def test_batch_errors(spm, torch):
 
 @pytest.mark.parametrize("N", [64, 96, 128])
 def test_every_shape_matches_oracle(spm, seeds, po, torch, N):
-    """Each FFT pass shape of N (different radices and tiles) against the oracle."""
+    """Each FFT pass shape of N (different radices and tiles) against the oracle,
+    in the axis-by-axis step the shapes define (step_passes = 4; at N = 128 the
+    default step is the two-pass one, tested in test_gpu_two_pass.py)."""
     steps = 6
     phi = seeds.crystal(N, "bcc", 26.0, 2 * np.pi, 0.1, 3)
     o = po.Oracle(N, kscale=2 * np.pi)
@@ -126,7 +128,7 @@ def test_every_shape_matches_oracle(spm, seeds, po, torch, N):
     shapes = spm.shapes(N)
     assert len(shapes) >= 2
     for k in range(len(shapes)):
-        with spm.Grid(N, 0, kscale=2 * np.pi, shape=k) as g:
+        with spm.Grid(N, 0, kscale=2 * np.pi, shape=k, step_passes=4) as g:
             t = torch.as_tensor(phi).cuda()
             itg, hist = g.gradflow(t, steps, -1.0, 0.0, check_every=steps)
             out = t.cpu().numpy()

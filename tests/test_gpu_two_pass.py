@@ -1,14 +1,16 @@
-"""The two-pass gradient-flow step of a 256^3 grid (the j transform split
-between the row pass and the i pass; the default at N = 256) against the
-oracle, against the four-pass step, in batches, and through its own
-state layout.  Tolerances as for every get_min_u_F check: FP64 energy 1e-12
-relative; FP32 against FP64 1e-6 relative."""
+"""The two-pass gradient-flow step and operator round trips of 128^3 and 256^3
+grids (the j transform split between the row pass and the i pass; the
+default at N = 128 and 256) against the oracle, against the axis-by-axis passes, in
+batches, and through their own state layout.  Tolerances as for every
+get_min_u_F check: FP64 energy 1e-12 relative; FP32 against FP64 1e-6
+relative."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-N = 256
+NS = [128, 256]
+TWO = dict(step_passes=2)
 
 
 @pytest.fixture(scope="module")
@@ -19,21 +21,22 @@ def torch():
     return _t
 
 
-def _dev_seed(seeds, torch, kind, s, dtype, kscale=1.0):
+def _dev_seed(seeds, torch, N, kind, s, dtype, kscale=1.0):
     return seeds.device_field(torch, N, kind, 26.0, kscale, 0.1, s, dtype=dtype)
 
 
-def test_two_pass_gradflow_matches_oracle(spm, po, seeds, torch):
-    """get_min_u_F (LPsrc.cpp:40-117) at 256^3, FP64: per-step E, ||grad F||,
-    residual and the final field against the oracle."""
-    print("[case] two-pass gradflow 256 vs oracle", flush=True)
+@pytest.mark.parametrize("N", NS)
+def test_two_pass_gradflow_matches_oracle(spm, po, seeds, torch, N):
+    """get_min_u_F (LPsrc.cpp:40-117), FP64: per-step E, ||grad F||, residual
+    and the final field against the oracle."""
+    print(f"[case] two-pass gradflow {N} vs oracle", flush=True)
     steps = 4
     phi = seeds.icosahedral_nucleus(N, seed=0)
     o = po.Oracle(N)
     uR = phi.astype(np.complex128)
     uF = o.my_fft(uR)
     it, E, res, gn = o.get_min_u_F(uR, uF, steps, 1e-16, gnorm=True)
-    with spm.Grid(N, 0) as g:
+    with spm.Grid(N, 0, **TWO) as g:
         pt = torch.as_tensor(phi).cuda()
         itg, hist = g.gradflow(pt, steps, 1e-16, 0.0, check_every=2)
         out = pt.cpu().numpy()
@@ -44,16 +47,17 @@ def test_two_pass_gradflow_matches_oracle(spm, po, seeds, torch):
     assert np.abs(out - uR.real).max() <= 1e-10
 
 
+@pytest.mark.parametrize("N", NS)
 @pytest.mark.parametrize("prec", [0, 1])
-def test_two_pass_equals_four_pass(spm, seeds, torch, prec):
+def test_two_pass_equals_four_pass(spm, seeds, torch, N, prec):
     """Same iterations as the axis-by-axis step: FP64 to rounding (1e-13),
     FP32 to its own rounding (1e-6 relative energy, 1e-5 field)."""
-    print(f"[case] two vs four passes prec={prec}", flush=True)
+    print(f"[case] two vs four passes N={N} prec={prec}", flush=True)
     steps = 40
     out = {}
     for passes in (2, 4):
         with spm.Grid(N, prec, kscale=2 * np.pi, Tau=0.005, step_passes=passes) as g:
-            t = _dev_seed(seeds, torch, "bcc", 3, g.dtype, 2 * np.pi)
+            t = _dev_seed(seeds, torch, N, "bcc", 3, g.dtype, 2 * np.pi)
             it, hist = g.gradflow(t, steps, -1.0, 0.0, check_every=16)
             out[passes] = (it, hist.copy(), t.double().cpu().numpy())
     (i2, h2, f2), (i4, h4, f4) = out[2], out[4]
@@ -64,13 +68,14 @@ def test_two_pass_equals_four_pass(spm, seeds, torch, prec):
     assert np.abs(f2 - f4).max() <= tol_f * np.abs(f4).max()
 
 
+@pytest.mark.parametrize("N", NS)
 @pytest.mark.parametrize("prec", [0, 1])
-def test_two_pass_layout_round_trip(spm, seeds, torch, prec):
+def test_two_pass_layout_round_trip(spm, seeds, torch, N, prec):
     """gf_begin then gf_end with no step: phi -> (row pass, i pass) -> state
     -> (i pass, row pass) -> phi, through the two-pass state layout."""
-    print(f"[case] two-pass round trip prec={prec}", flush=True)
-    with spm.Grid(N, prec) as g:
-        phi = _dev_seed(seeds, torch, "nucleus", 1, g.dtype)
+    print(f"[case] two-pass round trip N={N} prec={prec}", flush=True)
+    with spm.Grid(N, prec, **TWO) as g:
+        phi = _dev_seed(seeds, torch, N, "nucleus", 1, g.dtype)
         g.gf_begin(phi, 10, 1e-16, 0.0)
         back = torch.empty_like(phi)
         it, _ = g.gf_end(back, 10)
@@ -79,14 +84,15 @@ def test_two_pass_layout_round_trip(spm, seeds, torch, prec):
     assert err < (64 * 2e-15 if prec == 0 else 3e-6)
 
 
+@pytest.mark.parametrize("N", NS)
 @pytest.mark.parametrize("prec", [0, 1])
-def test_two_pass_batch_equals_single_runs(spm, seeds, torch, prec):
+def test_two_pass_batch_equals_single_runs(spm, seeds, torch, N, prec):
     """Batch members (own Tau, Gamma, limits) give exactly the single runs."""
-    print(f"[case] two-pass batch prec={prec}", flush=True)
+    print(f"[case] two-pass batch N={N} prec={prec}", flush=True)
     members = [("bcc", -0.02, 0.20, 21), ("fcc", 0.01, 0.40, 30)]
-    kw = dict(kscale=2 * np.pi)
+    kw = dict(kscale=2 * np.pi, **TWO)
     with spm.Batch(N, prec, capacity=2, hist_cap=64, **kw) as b:
-        phis = [_dev_seed(seeds, torch, k, i, b.dtype, 2 * np.pi) for i, (k, *_) in enumerate(members)]
+        phis = [_dev_seed(seeds, torch, N, k, i, b.dtype, 2 * np.pi) for i, (k, *_) in enumerate(members)]
         for m, ((_, Tau, Gamma, mi), t) in enumerate(zip(members, phis)):
             b.load(m, t, Tau, Gamma, mi, 1e-16, 0.0)
         b.steps(32)
@@ -107,31 +113,33 @@ def _rel(a, b):
     return (a - b).abs().max().item() / max(b.abs().max().item(), 1e-300)
 
 
-def test_two_pass_ngrad_matches_oracle(spm, po, seeds, torch):
-    """ngrad_cam (LPsrc.cpp:119-173) at 256^3 through the two-pass round trip,
-    FP64, against the oracle (1e-10 max norm, as at the other sizes)."""
-    print("[case] two-pass ngrad 256 vs oracle", flush=True)
+@pytest.mark.parametrize("N", NS)
+def test_two_pass_ngrad_matches_oracle(spm, po, seeds, torch, N):
+    """ngrad_cam (LPsrc.cpp:119-173) through the two-pass round trip, FP64,
+    against the oracle (1e-10 max norm, as at the other sizes)."""
+    print(f"[case] two-pass ngrad {N} vs oracle", flush=True)
     phi = seeds.icosahedral_nucleus(N, seed=2)
     o = po.Oracle(N)
     g_ref = o.ngrad_cam(phi + 0j).real
-    with spm.Grid(N, 0) as g:
+    with spm.Grid(N, 0, **TWO) as g:
         gr, nrm = g.ngrad(torch.as_tensor(phi).cuda())
         gr = gr.cpu().numpy()
     assert np.abs(gr - g_ref).max() <= 1e-10
     assert abs(nrm - o.nrmp(g_ref + 0j)) <= 1e-10 * nrm
 
 
+@pytest.mark.parametrize("N", NS)
 @pytest.mark.parametrize("prec", [0, 1])
-def test_two_pass_operators_equal_four_pass(spm, seeds, torch, prec):
+def test_two_pass_operators_equal_four_pass(spm, seeds, torch, N, prec):
     """hv_cam, precond_camnew, ngrad_cam (LOBPCG.cpp:113-199, LPsrc.cpp:119-173)
     and the SPM forces (SPM.cpp:34-168) through the two-pass round trip equal
     the axis-by-axis passes to rounding (FP64 1e-13, FP32 2e-6 relative)."""
-    print(f"[case] two-pass operators prec={prec}", flush=True)
+    print(f"[case] two-pass operators N={N} prec={prec}", flush=True)
     tol = 1e-13 if prec == 0 else 2e-6
     out = {}
     for passes in (2, 4):
         with spm.Grid(N, prec, step_passes=passes) as g:
-            phi = _dev_seed(seeds, torch, "nucleus", 4, g.dtype)
+            phi = _dev_seed(seeds, torch, N, "nucleus", 4, g.dtype)
             gen = torch.Generator(device="cuda").manual_seed(5)
             v = (torch.rand((N, N, N), generator=gen, dtype=torch.float64, device="cuda") - 0.5).to(g.dtype)
             r = {"hv": g.hv(phi, v).clone(), "precond": g.precond(v.clone())}
@@ -146,15 +154,16 @@ def test_two_pass_operators_equal_four_pass(spm, seeds, torch, prec):
     assert abs(out[2]["nrm"] - out[4]["nrm"]) <= tol * out[4]["nrm"]
 
 
-def test_two_pass_lobpcg_and_spm_equal_four_pass(spm, seeds, torch):
-    """hiosd_lobpcg_inip and Spring_Pairing at 256^3 over the two-pass round
+@pytest.mark.parametrize("N", NS)
+def test_two_pass_lobpcg_and_spm_equal_four_pass(spm, seeds, torch, N):
+    """hiosd_lobpcg_inip and Spring_Pairing over the two-pass round
     trips follow the axis-by-axis runs (Ritz values 1e-9, SPM residual 1e-9
     relative after 6 iterations)."""
-    print("[case] two-pass lobpcg/spm", flush=True)
+    print(f"[case] two-pass lobpcg/spm N={N}", flush=True)
     out = {}
     for passes in (2, 4):
         with spm.Grid(N, 0, step_passes=passes) as g:
-            phi = _dev_seed(seeds, torch, "nucleus", 0, g.dtype)
+            phi = _dev_seed(seeds, torch, N, "nucleus", 0, g.dtype)
             gen = torch.Generator(device="cuda").manual_seed(0)
             start = torch.rand((4, N, N, N), generator=gen, dtype=torch.float64, device="cuda") * 2 - 1
             V = g.lobpcg_init(start)
