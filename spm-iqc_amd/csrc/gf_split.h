@@ -52,18 +52,19 @@ template <typename T, int N> struct SJ {
         i = P / (DL / (2 * S1));
         lo = 2 * (P % (DL / (2 * S1))) + s;
     }
-    // pass 2: (h, ko, half kg of the ki): KB = KT/2 ki, NC = DL KB columns (l, ka);
-    // block b and b + 8 take kg = 0 and 1 of one (h, ko)
-    // CTPP threads per i pencil, DL values each (one register array serves the
-    // line role, DL values of l, and the column role, DL values of i)
+    // pass 2: (h, ko, group kg of KB ki), NC = DL KB columns (l, ka); the NSIB =
+    // KT / KB blocks of one (h, ko) share its W lines and are 8 block indices
+    // apart.  CTPP threads per i pencil, DL values each (one register array
+    // serves the line role, DL values of l, and the column role, DL values of i)
     static constexpr int KB = KT / 2;
+    static constexpr int NSIB = KT / KB;
     static constexpr int NC = DL * KB;
     static constexpr int CTPP = N / DL;
     static constexpr int C_THREADS = NC * CTPP;
-    static constexpr long C_BLOCKS = 2L * DH * KO;
+    static constexpr long C_BLOCKS = (long)NSIB * DH * KO;
     __device__ static void cblock(int b, int &h, int &ko, int &kg)
     {
-        const int x = b % 8, s = (b / 8) % 2, q = b / 16;
+        const int x = b % 8, s = (b / 8) % NSIB, q = b / (8 * NSIB);
         const int P = q * 8 + x; // (h, ko)
         h = P / KO;
         ko = P % KO;
@@ -72,7 +73,7 @@ template <typename T, int N> struct SJ {
     // U: the state of one pass-2 block is one contiguous chunk [i][l][ka]
     __host__ __device__ static constexpr long uof(int h, int ko, int kg, int i, int c)
     {
-        return ((((long)h * KO + ko) * 2 + kg) * N + i) * NC + c;
+        return ((((long)h * KO + ko) * NSIB + kg) * N + i) * NC + c;
     }
     static_assert(R_THREADS == S1 * (N / 2), "pass 1: one (l, kz < N/2) column per thread");
     static_assert(C_THREADS == N * KB, "pass 2: one (i, ka) line per thread");
