@@ -37,7 +37,7 @@ def algorithmic_bytes_per_step(N: int, s: int) -> int:
     return 2 * R * s + 5 * C * 2 * s
 
 
-# the kernels of one step: two passes (N = 128, 256) or four
+# the kernels of one step: two passes (N = 64, 128, 256) or four
 STEP_KERNELS_2 = (("k_sj_cols", "SOpGradFlowStep"), ("k_sj_rows", "ProSJFinalize"))
 STEP_KERNELS_4 = (("k_cols", "SOpGradFlowStep"), ("k_cols", "ProGradFlowReduce"), ("k_rows", "ProGradFlowFinalize"),
                   ("k_cols", "true, false, true, SOpNone, ProNone"))
@@ -145,7 +145,7 @@ def main():
                     help="FFT pass shape index (threads per pencil of the three passes), or FP64,FP32 indices; "
                          "-1: measure them all first and use the fastest (the tuner; only for the four-pass step)")
     ap.add_argument("--step-passes", type=int, default=0, choices=(0, 2, 4),
-                    help="array passes per step: 0 = the library default (2 at N = 128 and 256, else 4)")
+                    help="array passes per step: 0 = the library default (2 at N = 64, 128, 256, else 4)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests of the serving stream (p50/p99 latency); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=12.0)
@@ -169,7 +169,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     shapes = spm.shapes(N)
-    passes = args.step_passes or (2 if N in (128, 256) else 4)
+    passes = args.step_passes or (2 if N in (64, 128, 256) else 4)
     tuned = {}
     want = [int(x) for x in str(args.shape).split(",")]
     want = want * 2 if len(want) == 1 else want

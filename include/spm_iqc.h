@@ -54,10 +54,10 @@ typedef struct {
                        (spm_shapes), -1 = the tuned one (spm_tune; measured on
                        first use and kept in the tuning cache) */
     int step_passes; /* array passes per gradient-flow step (and per transform
-                       round trip of the operators): 0 = default (2 at N = 128
-                       and 256, 4 otherwise), 4 = one pass per axis (any N),
-                       2 = the j transform split between the row and i passes
-                       (N = 128 and 256 only) */
+                       round trip of the operators): 0 = default (2 at N = 64,
+                       128 and 256, 4 otherwise), 4 = one pass per axis (any
+                       N), 2 = the j transform split between the row and i
+                       passes (N = 64, 128 and 256 only) */
 } spm_params;
 
 typedef struct {

@@ -1,4 +1,4 @@
-// gf_split.h -- the gradient-flow step of a 128^3 or 256^3 grid in two passes
+// gf_split.h -- the gradient-flow step of a 64^3, 128^3 or 256^3 grid in two passes
 // over the work array instead of four (gfx950, HIP).
 //
 // The length-N transform along j is split by one Cooley-Tukey step,
@@ -27,7 +27,7 @@
 namespace spm {
 
 template <typename T, int N> struct SJ {
-    static_assert(N == 128 || N == 256, "two-pass gradient-flow step: N = 128 or 256");
+    static_assert(N == 64 || N == 128 || N == 256, "two-pass gradient-flow step: N = 64, 128 or 256");
     static constexpr int NH = N / 2 + 1;                // valid kz
     static constexpr int DL = 16, DH = N / DL;          // ranges of the digits h and l: j = DL h + l
     static constexpr int KT = sizeof(T) == 8 ? 2 : 4;   // kz per contiguous group
@@ -61,14 +61,17 @@ template <typename T, int N> struct SJ {
     static constexpr int NC = DL * KB;
     static constexpr int CTPP = N / DL;
     static constexpr int C_THREADS = NC * CTPP;
-    static constexpr long C_BLOCKS = (long)NSIB * DH * KO;
-    __device__ static void cblock(int b, int &h, int &ko, int &kg)
+    static constexpr int NPG = (DH * KO + 7) / 8 * 8; // (h, ko) pairs, padded to whole XCD rounds
+    static constexpr long C_BLOCKS = (long)NSIB * NPG;
+    // returns false for the padding pairs (no work; zero partial sums)
+    __device__ static bool cblock(int b, int &h, int &ko, int &kg)
     {
         const int x = b % 8, s = (b / 8) % NSIB, q = b / (8 * NSIB);
         const int P = q * 8 + x; // (h, ko)
-        h = P / KO;
+        h = P < DH * KO ? P / KO : 0;
         ko = P % KO;
         kg = s;
+        return P < DH * KO;
     }
     // U: the state of one pass-2 block is one contiguous chunk [i][l][ka]
     __host__ __device__ static constexpr long uof(int h, int ko, int kg, int i, int c)
@@ -77,7 +80,7 @@ template <typename T, int N> struct SJ {
     }
     static_assert(R_THREADS == S1 * (N / 2), "pass 1: one (l, kz < N/2) column per thread");
     static_assert(C_THREADS == N * KB, "pass 2: one (i, ka) line per thread");
-    static_assert((DH * KO) % 8 == 0 && (N * DL / (2 * S1)) % 8 == 0, "block maps");
+    static_assert((N * DL / (2 * S1)) % 8 == 0, "pass-1 block map");
 };
 
 // ----------------------------------------------------------------------
@@ -410,16 +413,16 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
     if (dst) dst += mb * bs;
     load_twiddles<T, N>(stw, tw);
     int h, ko, kg;
-    S::cblock(blockIdx.x, h, ko, kg);
+    const bool real_blk = S::cblock(blockIdx.x, h, ko, kg);
     // line role: thread (ia, ka) holds the DL values l of (i = ia, ki = KB kg + ka)
     const int ka = threadIdx.x % KB, ia = threadIdx.x / KB;
     const int kia = KB * kg + ka;
-    const bool va = ko * KT + kia < S::NH;
+    const bool va = real_blk && ko * KT + kia < S::NH;
     const long lbase = S::wof(ia, h, ko, 0, kia);
     // column role: thread (cb, tb) holds column cb = (l, ka) at i = tb + TPP m
     const int cb = threadIdx.x % NC, tb = threadIdx.x / NC;
     const int lb = cb / KB, kzb = ko * KT + KB * kg + cb % KB;
-    const bool vb = kzb < S::NH;
+    const bool vb = real_blk && kzb < S::NH;
     const long ub = S::uof(h, ko, kg, 0, 0);
     C2<T> v[E];
     static_assert(E == DL, "one register array serves both roles");

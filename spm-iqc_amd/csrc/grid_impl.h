@@ -59,7 +59,7 @@ struct GridImpl final : GridIface {
                            1, (const CT *)c.tw, op, part, ctl, ProNone{}, bs);
     }
 
-    // two-pass gradient-flow step (N = 128, 256): pass 1 over rows, pass 2 over columns
+    // two-pass gradient-flow step (N = 64, 128, 256): pass 1 over rows, pass 2 over columns
     using SJT = SJ<T, N>;
     template <bool IN_SPEC, bool OUT_SPEC, class Op, class Pro = ProNone>
     static void sj_rows(Ctx &c, const void *in, void *out, Op op, double *part, const Ctl *ctl, Pro pro = Pro{},
@@ -79,7 +79,7 @@ struct GridImpl final : GridIface {
     }
 
     // A transform round trip r2c -> spectral op -> c2r as pass 1, pass 2, pass 1
-    // (N = 128, 256 in two-pass mode): six array passes instead of ten.  opin runs on
+    // (N = 64, 128, 256 in two-pass mode): six array passes instead of ten.  opin runs on
     // the input field (its sums into pin), opout on the output (sums into pout);
     // mid (host) runs between the first pass and the second (scalar reductions).
     template <class OpIn, class SOp, class OpOut, class Mid>
@@ -91,7 +91,7 @@ struct GridImpl final : GridIface {
         sj_cols<true, true>(c, c.W, c.W, sop, nullptr, nullptr);
         sj_rows<true, false>(c, c.W, nullptr, opout, pout, nullptr);
     }
-    static bool two_pass(const Ctx &c) { return (N == 128 || N == 256) && c.gf2; }
+    static bool two_pass(const Ctx &c) { return (N == 64 || N == 128 || N == 256) && c.gf2; }
 
     // ---------------------------------------------------------- transforms
     void r2c(Ctx &c, const void *x, void *S) override
@@ -143,7 +143,7 @@ struct GridImpl final : GridIface {
     // ngrad_cam (LPsrc.cpp:119-173)
     void ngrad(Ctx &c, const void *phi, void *grad) override
     {
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
                 sj_round_trip(
                     c, phi, OpNgradIn<T>{c.Tau, c.Gamma}, c.slot(0),
@@ -167,7 +167,7 @@ struct GridImpl final : GridIface {
     // hv_cam (LOBPCG.cpp:113-166)
     void hv(Ctx &c, const void *phi, const void *v, void *hvo) override
     {
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
                 sj_round_trip(
                     c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0),
@@ -191,7 +191,7 @@ struct GridImpl final : GridIface {
     // minus its mean (= the same with the DC set to zero).
     void precond(Ctx &c, void *w) override
     {
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
                 sj_round_trip(c, w, OpNone{}, nullptr, [] {}, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab},
                               OpStore<T>{(T *)w}, nullptr);
@@ -207,7 +207,7 @@ struct GridImpl final : GridIface {
 
     void gconv(Ctx &c, const void *in, void *out) override
     {
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
                 sj_round_trip(c, in, OpNone{}, nullptr, [] {}, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
                               OpStore<T>{(T *)out}, nullptr);
@@ -223,7 +223,7 @@ struct GridImpl final : GridIface {
 
     void force(Ctx &c, void *x, const void *v, const double *s, double coef) override
     {
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
                 sj_round_trip(c, x, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr, [] {},
                               SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, OpStore<T>{(T *)x}, nullptr);
@@ -248,7 +248,7 @@ struct GridImpl final : GridIface {
     void gf_init_member(Ctx &c, int m, const void *phi, double Tau, double Gamma) override
     {
         CT *Um = (CT *)c.U + (long)m * c.C, *Wm = (CT *)c.W + (long)m * c.C;
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (c.gf2) { // U and W in the two-pass layout; slot 2 holds the density partials of phi
                 using S = SJT;
                 sj_rows<false, true>(c, phi, Wm, OpNone{}, nullptr, c.ctl + m);
@@ -267,7 +267,7 @@ struct GridImpl final : GridIface {
     // phi of member m from its state spectrum (its W is the scratch)
     void gf_read_member(Ctx &c, int m, void *phi) override
     {
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (c.gf2) {
                 CT *Um = (CT *)c.U + (long)m * c.C, *Wm = (CT *)c.W + (long)m * c.C;
                 sj_cols<false, true>(c, Um, Wm, SOpNone{}, nullptr, nullptr);
@@ -289,7 +289,7 @@ struct GridImpl final : GridIface {
         const int nm = c.nmem;
         const long bs = c.C;
         const bool nt = 2.0 * (double)bs * sizeof(CT) * nm > 200e6; // W + U beyond the 256 MB last-level cache
-        if constexpr (N == 128 || N == 256) {
+        if constexpr (N == 64 || N == 128 || N == 256) {
             if (c.gf2) {
                 using S = SJT;
                 sj_cols<true, true>(c, c.W, c.W,

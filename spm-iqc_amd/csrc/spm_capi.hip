@@ -218,8 +218,8 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
         return spm_fail(SPM_EINVAL, "spm_create: L, kscale, dt must be > 0 and Tau, Gamma finite");
     if (p->step_passes != 0 && p->step_passes != 2 && p->step_passes != 4)
         return spm_fail(SPM_EINVAL, "spm_create: step_passes must be 0, 2 or 4");
-    if (p->step_passes == 2 && p->N != 128 && p->N != 256)
-        return spm_fail(SPM_ENOTSUP, "spm_create: the two-pass gradient-flow step exists for N = 128 and 256 only");
+    if (p->step_passes == 2 && p->N != 64 && p->N != 128 && p->N != 256)
+        return spm_fail(SPM_ENOTSUP, "spm_create: the two-pass gradient-flow step exists for N = 64, 128 and 256 only");
     spm_ctx *h = new spm_ctx();
     Ctx &c = h->c;
     c.N = p->N;
@@ -250,7 +250,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
                                         std::to_string(c.N));
     }
     c.shape = shape;
-    c.gf2 = p->step_passes == 2 || (p->step_passes == 0 && (c.N == 128 || c.N == 256));
+    c.gf2 = p->step_passes == 2 || (p->step_passes == 0 && (c.N == 64 || c.N == 128 || c.N == 256));
     c.nmem = nmem;
     c.g = make_grid(c.N, c.prec, shape);
     if (!c.g) {

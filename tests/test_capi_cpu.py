@@ -57,12 +57,12 @@ def test_seeds(seeds):
 
 def test_step_passes_validated_before_any_device_work(spm):
     """spm_create rejects a bad step_passes value (and the two-pass step at an N
-    other than 128 and 256) during parameter checks, before it touches a device."""
+    other than 64, 128 and 256) during parameter checks, before it touches a device."""
     import ctypes
 
     lib = spm.lib()
     h = ctypes.c_void_p()
-    for N, passes, want in ((64, 3, -1), (64, 2, -4), (192, 2, -4), (256, -1, -1)):
+    for N, passes, want in ((64, 3, -1), (96, 2, -4), (192, 2, -4), (256, -1, -1)):
         p = spm._params(N, spm.FP64, 26.0, 1.0, -0.0061, 0.30, 0.3, 0, passes)
         assert lib.spm_create(ctypes.byref(p), ctypes.byref(h)) == want
         assert not h.value

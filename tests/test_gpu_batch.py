@@ -116,8 +116,8 @@ def test_batch_errors(spm, torch):
 @pytest.mark.parametrize("N", [64, 96, 128])
 def test_every_shape_matches_oracle(spm, seeds, po, torch, N):
     """Each FFT pass shape of N (different radices and tiles) against the oracle,
-    in the axis-by-axis step the shapes define (step_passes = 4; at N = 128 the
-    default step is the two-pass one, tested in test_gpu_two_pass.py)."""
+    in the axis-by-axis step the shapes define (step_passes = 4; at N = 64 and
+    128 the default step is the two-pass one, tested in test_gpu_two_pass.py)."""
     steps = 6
     phi = seeds.crystal(N, "bcc", 26.0, 2 * np.pi, 0.1, 3)
     o = po.Oracle(N, kscale=2 * np.pi)

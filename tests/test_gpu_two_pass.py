@@ -1,6 +1,6 @@
-"""The two-pass gradient-flow step and operator round trips of 128^3 and 256^3
+"""The two-pass gradient-flow step and operator round trips of 64^3, 128^3 and 256^3
 grids (the j transform split between the row pass and the i pass; the
-default at N = 128 and 256) against the oracle, against the axis-by-axis passes, in
+default at N = 64, 128 and 256) against the oracle, against the axis-by-axis passes, in
 batches, and through their own state layout.  Tolerances as for every
 get_min_u_F check: FP64 energy 1e-12 relative; FP32 against FP64 1e-6
 relative."""
@@ -9,7 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-NS = [128, 256]
+NS = [64, 128, 256]
 TWO = dict(step_passes=2)
 
 
