@@ -23,6 +23,13 @@ struct GridImpl final : GridIface {
         nb_j = (long)N * ((NHP + GJ::B - 1) / GJ::B);
         nb_i = ((long)N * NHP + GI::B - 1) / GI::B;
     }
+    long two_pass_partials() override
+    {
+        if constexpr (N == 64 || N == 128 || N == 256)
+            return std::max<long>(SJ<T, N>::R_BLOCKS, 3 * SJ<T, N>::C_BLOCKS);
+        else
+            return 0;
+    }
 
     // ---------------------------------------------------------- launchers
     // nm > 1: nm batch members (blockIdx.y), member m at in + m * in_bs, out + m * out_bs

@@ -63,6 +63,8 @@ struct Ctx {
 struct GridIface {
     virtual ~GridIface() = default;
     virtual void geometry(long &nb_rows, long &nb_j, long &nb_i, int NHP) = 0;
+    // partial sums one field writes in a two-pass launch (0: no two-pass kernels for this N)
+    virtual long two_pass_partials() = 0;
     // S = FFT(x) / N^3 into the half spectrum S
     virtual void r2c(Ctx &c, const void *x, void *S) = 0;
     // x = IFFT(S) (unnormalised); S is preserved (W is used)
