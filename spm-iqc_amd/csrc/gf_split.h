@@ -303,63 +303,37 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     }
     if constexpr (OUT_SPEC) {
         fft_pencil_split<T, N, TPP, -1, 1, true>(v, my, stw, t);
-        // separation of the two rows: needs Z[k] and Z[N - k] (another thread's)
-        constexpr int EH = E / 2 + 1;
-        C2<T> xa[EH], xb[EH];
+        // The column role separates the two rows of a pair itself: pair pp = h
+        // holds rows (h, l even) and (h, l odd), and with Z its spectrum,
+        //   Xa[k] = (Z[k] + conj Z[N-k]) / 2,  Xb[k] = -i (Z[k] - conj Z[N-k]) / 2.
+        // Z goes through LDS as real parts, then imaginary parts; the kz = N/2
+        // entries (real) go to nyq.
+        static_assert(S1 == 2, "pair h = rows (h, 2 lo), (h, 2 lo + 1)");
+        T zr[DH], zs[DH];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].x;
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < DH; ++h) {
+            zr[h] = lds[h * RSTR + kz];
+            zs[h] = lds[h * RSTR + (N - kz) % N];
+        }
+        if (tid < DH) nyq[tid * S1] = lds[tid * RSTR + N / 2]; // Xa[N/2] = Re Z[N/2]
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].y;
+        __syncthreads();
         {
-            T ax[EH], bx[EH];
-            __syncthreads();
+            const T hx = (T)0.5;
 #pragma unroll
-            for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].x;
-            __syncthreads();
-#pragma unroll
-            for (int m = 0; m < EH; ++m) {
-                const int k = t + TPP * m;
-                ax[m] = k <= N / 2 ? my[k] : (T)0;
-                bx[m] = k <= N / 2 ? my[(N - k) % N] : (T)0;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].y;
-            __syncthreads();
-#pragma unroll
-            for (int m = 0; m < EH; ++m) {
-                const int k = t + TPP * m;
-                const T ay = k <= N / 2 ? my[k] : (T)0, by = k <= N / 2 ? my[(N - k) % N] : (T)0;
-                // A = Z[k], B = Z[N-k]: Xa = (A + conj B)/2, Xb = -i (A - conj B)/2
-                const T hx = (T)0.5;
-                xa[m] = C2<T>{hx * (ax[m] + bx[m]), hx * (ay - by)};
-                xb[m] = C2<T>{hx * (ay + by), hx * (bx[m] - ax[m])};
+            for (int h = 0; h < DH; ++h) {
+                const T zi = lds[h * RSTR + kz], zj = lds[h * RSTR + (N - kz) % N];
+                c[h] = li == 0 ? C2<T>{hx * (zr[h] + zs[h]), hx * (zi - zj)} : C2<T>{hx * (zi + zj), hx * (zs[h] - zr[h])};
             }
         }
+        if (tid >= DH && tid < 2 * DH) nyq[(tid - DH) * S1 + 1] = lds[(tid - DH) * RSTR + N / 2]; // Xb[N/2] = Im Z[N/2]
         __syncthreads();
-        // rows -> columns through the row buffer: real parts, then imaginary parts
-#pragma unroll
-        for (int m = 0; m < EH; ++m) {
-            const int k = t + TPP * m;
-            if (k < N / 2) {
-                lds[ra * RS + k] = xa[m].x;
-                lds[rb * RS + k] = xb[m].x;
-            } else if (k == N / 2) {
-                nyq[ra] = xa[m].x;
-                nyq[rb] = xb[m].x;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < DH; ++h) c[h].x = lds[(h * S1 + li) * RS + kz];
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < EH; ++m) {
-            const int k = t + TPP * m;
-            if (k < N / 2) {
-                lds[ra * RS + k] = xa[m].y;
-                lds[rb * RS + k] = xb[m].y;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < DH; ++h) c[h].y = lds[(h * S1 + li) * RS + kz];
         // A: 16-point DFT over h, then the twiddle; digit k1 goes to position h
         C2<T> *Wo = (C2<T> *)out + mb * out_bs;
         auto stage_a = [&](int lc) {
