@@ -246,8 +246,9 @@ struct GridImpl final : GridIface {
 
     // ---------------------------------------------------------- gradient flow
     // State: U = u_F (normalised spectrum of phi).  W carries the spectrum of
-    // f(phi) between passes.  Slot 2 holds the E2 sums of the current phi
-    // (row pass), slot 3 the E1 / ||grad F||^2 / residual sums (i pass).
+    // f(phi) between passes (both in the two-pass layouts when c.gf2).  Slot 2
+    // holds the E2 sums of the current phi (row pass), slot 3 the E1 /
+    // ||grad F||^2 / residual sums (i pass).
     void gf_init(Ctx &c, const void *phi) override { gf_init_member(c, 0, phi, c.Tau, c.Gamma); }
 
     // Member m of a batch (m = 0 for a single field): its U and W are the m-th
@@ -288,6 +289,11 @@ struct GridImpl final : GridIface {
     // One get_min_u_F iteration (LPsrc.cpp:52-111).  Kernels return at once
     // when ctl->stop is set, so a fixed sequence of steps can be queued (or
     // replayed from a graph) past convergence.
+    // Two passes (c.gf2, N = 64, 128, 256; gf_split.h): pass 2 (j stage B, i FFT,
+    // update, inverse) then pass 1 (A^-1, c2r, f(phi), r2c, A).  Block 0 of pass 2
+    // sums the density partials of the previous pass 1 into slot 4; block 0 of
+    // pass 1 closes the iteration from the pass-2 partials (slot 3).
+    // Four passes otherwise: i pass, j inverse (reduces), rows (closes), j forward.
     // Batched (c.nmem members): every launch covers all members, each with its
     // own (Tau, Gamma) from c.prm; members whose ctl[m].stop is set skip out.
     void gf_step(Ctx &c) override
