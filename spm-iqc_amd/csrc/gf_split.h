@@ -291,14 +291,21 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     // The pointwise work sits in a basic block of its own (the condition always
     // holds): scheduled in one region with the FFT code around it, it took
     // twice the registers.
-    if (blockIdx.x < gridDim.x) {
+    {
+        constexpr int CH = op_chunk<Op>::value > 0 && op_chunk<Op>::value < E ? op_chunk<Op>::value : E;
+        static_assert(E % CH == 0, "whole chunks");
         const int ja = DL * (ra / S1) + S1 * lo + ra % S1, jb = ja + 1;
         const long ga = ((long)i * N + ja) * N, gb = ((long)i * N + jb) * N;
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int p = t + TPP * m;
-            v[m].x = op(v[m].x, ga + p, acc);
-            v[m].y = op(v[m].y, gb + p, acc);
+        for (int m0 = 0; m0 < E; m0 += CH) {
+            if (blockIdx.x < gridDim.x + m0) {
+#pragma unroll
+                for (int m = m0; m < m0 + CH; ++m) {
+                    const int p = t + TPP * m;
+                    v[m].x = op(v[m].x, ga + p, acc);
+                    v[m].y = op(v[m].y, gb + p, acc);
+                }
+            }
         }
     }
     if constexpr (OUT_SPEC) {

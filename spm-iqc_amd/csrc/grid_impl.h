@@ -26,7 +26,7 @@ struct GridImpl final : GridIface {
     long two_pass_partials() override
     {
         if constexpr (N == 64 || N == 128 || N == 256)
-            return std::max<long>(SJ<T, N>::R_BLOCKS, 3 * SJ<T, N>::C_BLOCKS);
+            return std::max<long>(8 * SJ<T, N>::R_BLOCKS, 3 * SJ<T, N>::C_BLOCKS);
         else
             return 0;
     }
@@ -242,6 +242,59 @@ struct GridImpl final : GridIface {
         cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, nullptr);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
         rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+    }
+
+    long spm_tangent(Ctx &c, void *p0, void *p1, void *d, bool spring, double dt, double ks, double *pt,
+                     double *pf) override
+    {
+        const OpSpringIn<T> in{dt, ks, spring, (T *)p0, (T *)p1, (T *)d};
+        const OpFvOut<T> out{c.Tau, c.Gamma, (const T *)p0, (const T *)p1, (const T *)d};
+        const SOpMulTab<T, N> g{1.0 / (double)c.R, c.Gtab};
+        if constexpr (N == 64 || N == 128 || N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(c, p0, in, pt, [] {}, g, out, pf);
+                return SJT::R_BLOCKS;
+            }
+        }
+        rows<false, true>(c, p0, c.W, in, pt);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, g, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, out, pf);
+        return c.nb_rows;
+    }
+
+    // force, then ngrad_cam's norm at the updated x: the output pass of the
+    // force round trip is also the input pass of ngrad's (one pass over x fewer).
+    void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef) override
+    {
+        const double sc = 1.0 / (double)c.R;
+        const OpForceIn<T> fin{c.Tau, c.Gamma, coef, (const T *)v, s};
+        const OpStoreNgradIn<T> mid{c.Tau, c.Gamma, (T *)x};
+        const OpNgradOut<T> nout{c.Tau, c.Gamma, (const T *)x, c.scal + SC_MEAN_F, nullptr};
+        if constexpr (N == 64 || N == 128 || N == 256) {
+            if (two_pass(c)) {
+                sj_rows<false, true>(c, x, c.W, fin, nullptr, nullptr);
+                sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Btab}, nullptr, nullptr);
+                sj_rows<true, true>(c, c.W, c.W, mid, c.slot(0), nullptr);
+                launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, sc, c.scal + SC_MEAN_F);
+                sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr);
+                sj_rows<true, false>(c, c.W, nullptr, nout, c.slot(1), nullptr);
+                launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, sc, c.scal + SC_GN2);
+                return;
+            }
+        }
+        rows<false, true>(c, x, c.W, fin, nullptr);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Btab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, true>(c, c.W, c.W, mid, c.slot(0));
+        launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, sc, c.scal + SC_MEAN_F);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, nout, c.slot(1));
+        launch_reduce(c, c.slot(1), c.nb_rows, 1, 0u, sc, c.scal + SC_GN2);
     }
 
     // ---------------------------------------------------------- gradient flow

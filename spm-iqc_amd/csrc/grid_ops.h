@@ -73,6 +73,13 @@ template <class Op> __device__ __forceinline__ Op bind_member(const Op &op, int 
         return op;
 }
 
+// A real-space operator that gathers from other fields may set CHUNK: the row
+// passes then apply it in basic blocks of CHUNK values per thread (or of all the
+// values a thread holds, when those are fewer), so that the
+// scheduler cannot hoist every gather of the thread at once (registers).
+template <class Op, class = void> struct op_chunk : std::integral_constant<int, 0> {};
+template <class Op> struct op_chunk<Op, std::void_t<decltype(Op::CHUNK)>> : std::integral_constant<int, Op::CHUNK> {};
+
 // ======================================================================
 // Real-space operators applied to each real value inside the row pass.
 // op(x, g, acc): x = value at real-field index g; returns the value the pass
@@ -157,12 +164,86 @@ template <typename T> struct OpNgradIn {
     }
 };
 
+// The force round trip's output pass fused with ngrad_cam's input pass
+// (SPM.cpp:235-236): the c2r result x is stored, the sum of
+// f(x) = Tau x + Gamma x^2 - x^3 accumulated (its mean), and x goes on to the
+// forward transform.
+template <typename T> struct OpStoreNgradIn {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    T *out;
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    {
+        out[g] = x;
+        const double u = (double)x, u2 = u * u;
+        acc[0] += Tau * u + Gamma * u2 - u2 * u;
+        return x;
+    }
+};
+
+// Spring_Pairing's tangent (SPM.cpp:15-32, 203-218) as the input pass of its
+// G convolution.  x = Phi0 (the pass's input), Phi1 is gathered; with spring,
+// the spring half-steps of SPM.cpp:203-216 update both replicas in place
+// (Phi0 += dt ks (Phi1 - Phi0), then Phi1 += dt ks (Phi0' - Phi1)).  The
+// unnormalised tangent d = Phi1 - Phi0 is stored and transformed, and the sum
+// of d^2 accumulated (nrmp(d)^2).
+template <typename T> struct OpSpringIn {
+    static constexpr int NRED = 1;
+    static constexpr unsigned MAXM = 0;
+    double dt, ks;
+    bool spring;
+    T *p0, *p1, *d;
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    {
+        T a1 = x, b1 = p1[g];
+        if (spring) {
+            const double a = (double)a1, b = (double)b1;
+            a1 = (T)(1 * a + dt * (ks * b + -ks * a));
+            b1 = (T)(1 * b + dt * (ks * (double)a1 + -ks * b));
+            p0[g] = a1;
+            p1[g] = b1;
+        }
+        const T di = b1 - a1;
+        d[g] = di;
+        acc[0] += (double)di * (double)di;
+        return di;
+    }
+};
+
+// The output pass of that G convolution: with w = (G*d)(g), the sums for
+// <f_r, d> of both replicas without forming f_r = ngrad_cam(Phi_r)
+// (LPsrc.cpp:147-169; <G*x, d> = <x, G*d> for the even kernel G):
+// per replica r the sums of x_r w, nl_r d and nl_r (nl = Tau x + Gamma x^2 - x^3),
+// then the sum of d.  G*d itself is not stored.
+template <typename T> struct OpFvOut {
+    static constexpr int NRED = 7;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma;
+    const T *x0, *x1, *d;
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    {
+        const double w = (double)x, vi = (double)d[g];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const double u = (double)(r == 0 ? x0 : x1)[g];
+            const double nl = Tau * u + Gamma * u * u - u * u * u;
+            acc[3 * r] += u * w;
+            acc[3 * r + 1] += nl * vi;
+            acc[3 * r + 2] += nl;
+        }
+        acc[6] += vi;
+        return x;
+    }
+};
+
 // ngrad_cam epilogue (LPsrc.cpp:147-169): x = ifft(G phi_hat, DC = 0);
 // grad = -((x - Tau u - Gamma u^2 + u^3) - mean), mean = -mean(f).
-// Sums grad^2 for nrmp.
+// Sums grad^2 for nrmp; grad = nullptr keeps only the sum.
 template <typename T> struct OpNgradOut {
     static constexpr int NRED = 1;
     static constexpr unsigned MAXM = 0;
+    static constexpr int CHUNK = 8;
     double Tau, Gamma;
     const T *phi;
     const double *mean_f; // device scalar: mean of f(phi)
@@ -172,7 +253,7 @@ template <typename T> struct OpNgradOut {
         const double u = (double)phi[g], u2 = u * u, u3 = u2 * u;
         const double gr = (double)x - Tau * u - Gamma * u2 + u3;
         const double out = -(gr + *mean_f);
-        grad[g] = (T)out;
+        if (grad) grad[g] = (T)out;
         acc[0] += out * out;
         return x;
     }

@@ -316,7 +316,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     if ((rc = spm_dev_alloc(h, &c.W, spec * nmem))) return bail(rc);
     if ((rc = spm_dev_alloc(h, &c.U, spec * nmem))) return bail(rc);
     long mb = std::max(std::max(c.nb_rows, c.nb_j), std::max(c.nb_i, (long)DOT_BLOCKS));
-    c.part_slot = std::max(4 * mb, c.g->two_pass_partials()) * nmem; // every member's share of a slot
+    c.part_slot = std::max(8 * mb, c.g->two_pass_partials()) * nmem; // every member's share of a slot
     c.nslots = 8;
     if ((rc = spm_dev_alloc(h, (void **)&c.part, sizeof(double) * c.part_slot * c.nslots))) return bail(rc);
     if ((rc = spm_dev_alloc(h, (void **)&c.scal, sizeof(double) * SC_COUNT))) return bail(rc);

@@ -16,7 +16,7 @@ namespace spm {
 struct GridIface;
 
 enum ScalarSlot { // device scalars (ctx.scal)
-    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_FV, SC_TV2, SC_RES0, SC_RES1,
+    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_FV, SC_TV2, SC_RES0, SC_RES1, SC_FV1,
     SC_COUNT = 64
 };
 
@@ -82,6 +82,13 @@ struct GridIface {
     // x <- IFFT(FFT(x + nln(x) - coef * (*s) * v) / N^3 / (1 + G)), DC = 0
     // (one force step of SPM.cpp:34-99 / 103-168; s a device scalar)
     virtual void force(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;
+    // Spring_Pairing's tangent d = Phi1 - Phi0 (after the spring half-steps on
+    // p0, p1 when spring): partial sums of d^2 into pt (one per block), the <f, d>
+    // sums of both replicas (OpFvOut, 7 per block) into pf; returns the block count
+    virtual long spm_tangent(Ctx &c, void *p0, void *p1, void *d, bool spring, double dt, double ks, double *pt,
+                             double *pf) = 0;
+    // force, then scal[SC_GN2] = nrmp(ngrad_cam(x))^2 of the updated x (no gradient field)
+    virtual void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;
     virtual void gf_init(Ctx &c, const void *phi) = 0;
     virtual void gf_init_member(Ctx &c, int m, const void *phi, double Tau, double Gamma) = 0;
     virtual void gf_read_member(Ctx &c, int m, void *phi) = 0;

@@ -28,7 +28,9 @@ __global__ void __launch_bounds__(256) k_dot2_fin(const double *__restrict__ par
 // Sums for <f, v> without forming f = ngrad_cam(x) (LPsrc.cpp:147-169):
 // f = -(G*x) + nl(x) - mean(nl), nl = Tau x + Gamma x^2 - x^3, and
 // <G*x, v> = <x, G*v> for the even kernel G (Parseval), so with vt = G*v
-// <f, v> = -mean(x vt) + mean(nl v) - mean(nl) mean(v).
+// <f, v> = -mean(x vt) + mean(nl v) - mean(nl) mean(v): the sums of x vt, nl v,
+// nl and v.  (Spring_Pairing gets the same sums of both replicas from the
+// output pass of its G convolution, OpFvOut.)
 template <typename T>
 __global__ void __launch_bounds__(256) k_fv_sums(const T *__restrict__ x, const T *__restrict__ v,
                                                 const T *__restrict__ vt, long n, double Tau, double Gamma,
@@ -50,15 +52,28 @@ __global__ void __launch_bounds__(256) k_fv_sums(const T *__restrict__ x, const 
         for (int q = 0; q < 4; ++q) part[(long)q * gridDim.x + blockIdx.x] = s[q];
 }
 
-__global__ void __launch_bounds__(256) k_fv_fin(const double *__restrict__ part, int nb, double R, double *out)
+// From NX replicas' sums (per replica x vt, nl v, nl; then v; part[q nb + b]),
+// out[r] = <f_r, v> / d2 (d2 = nrmp(v)^2 of an unnormalised tangent v, or 1
+// when d2 is null): the force term <f, t> t along the unit tangent t = v / nrmp(v)
+// is then out * v.
+template <int NX>
+__global__ void __launch_bounds__(256) k_fv_fin(const double *__restrict__ part, int nb, double R,
+                                               const double *d2, double *out0, double *out1)
 {
-    __shared__ double red[4 * 16];
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    constexpr int NS = 3 * NX + 1;
+    __shared__ double red[NS * 16];
+    double s[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) s[q] = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] += part[(long)q * nb + b];
-    block_reduce<4, 0u>(s, red);
-    if (threadIdx.x == 0) *out = (-s[0] + s[1]) / R - (s[2] / R) * (s[3] / R);
+        for (int q = 0; q < NS; ++q) s[q] += part[(long)q * nb + b];
+    block_reduce<NS, 0u>(s, red);
+    if (threadIdx.x == 0) {
+        const double w = d2 ? *d2 : 1.0, mv = s[3 * NX] / R;
+        *out0 = ((-s[0] + s[1]) / R - (s[2] / R) * mv) / w;
+        if (NX == 2) *out1 = ((-s[3] + s[4]) / R - (s[5] / R) * mv) / w;
+    }
 }
 
 // get_Tv (SPM.cpp:15-32), in two kernels: v = Phi1 - Phi0 (with its norm), v /= nrmp(v)
@@ -85,21 +100,9 @@ template <typename T> __global__ void k_div_by_sqrt(T *v, long n, const double *
         v[i] = (T)((double)v[i] / d);
 }
 
-// The spring half-step of SPM.cpp:203-216 (and 233-246), replicas updated in
-// the reference's order: Phi0 += dt ks (Phi1 - Phi0); Phi1 += dt ks (Phi0' - Phi1).
-template <typename T> __global__ void k_spring(T *p0, T *p1, long n, double dt, double ks)
-{
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double a = (double)p0[i], b = (double)p1[i];
-        const T a1 = (T)(1 * a + dt * (ks * b + -ks * a));
-        const T b1 = (T)(1 * b + dt * (ks * (double)a1 + -ks * b));
-        p0[i] = a1;
-        p1[i] = b1;
-    }
-}
-
 __global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
 
+// get_Tv: v = (Phi1 - Phi0) / nrmp(Phi1 - Phi0)
 template <typename T> int launch_tv(spm_ctx *h, const void *p0, const void *p1, void *v)
 {
     Ctx &c = h->c;
@@ -111,15 +114,24 @@ template <typename T> int launch_tv(spm_ctx *h, const void *p0, const void *p1, 
     return spm_launch_status("get_Tv");
 }
 
-int get_tv(spm_ctx *h, const void *p0, const void *p1, void *v)
+// Spring_Pairing's tangent (after the spring half-steps when spring): d = Phi1 -
+// Phi0 unnormalised, scal[SC_TV2] = nrmp(d)^2 and the force scalars
+// scal[SC_FV], scal[SC_FV1] = <f_r, d> / nrmp(d)^2 of both replicas, from one
+// G-convolution round trip of d (its passes carry the spring step and the sums).
+int tangent(spm_ctx *h, void *p0, void *p1, void *d, bool spring, double dt, double ks)
 {
-    return h->c.prec == SPM_FP64 ? launch_tv<double>(h, p0, p1, v) : launch_tv<float>(h, p0, p1, v);
+    Ctx &c = h->c;
+    double *pt = c.slot(5), *pf = c.slot(6);
+    const long nb = c.g->spm_tangent(c, p0, p1, d, spring, dt, ks, pt, pf);
+    if (7 * nb > c.part_slot) return spm_fail(SPM_EINVAL, "spm tangent: partial sums exceed a slot");
+    launch_reduce(c, pt, nb, 1, 0u, 1.0 / (double)c.R, c.scal + SC_TV2);
+    hipLaunchKernelGGL(k_fv_fin<2>, dim3(1), dim3(256), 0, c.stream, pf, (int)nb, (double)c.R, c.scal + SC_TV2,
+                       c.scal + SC_FV, c.scal + SC_FV1);
+    return spm_launch_status("spm tangent");
 }
 
-// Natural_Forces (coef 1) / MinMax_Forces (coef 2) on x in place, given the
-// tangent v and vt = G*v (one G convolution per tangent serves every force
-// with that tangent).
-int forces(spm_ctx *h, void *x, const void *v, const void *vt, double coef)
+// <f, v> of x along the unit tangent v into scal[SC_FV], given vt = G*v.
+int fv_scalar(spm_ctx *h, const void *x, const void *v, const void *vt)
 {
     Ctx &c = h->c;
     double *part = c.slot(6);
@@ -129,21 +141,9 @@ int forces(spm_ctx *h, void *x, const void *v, const void *vt, double coef)
     else
         hipLaunchKernelGGL(k_fv_sums<float>, dim3(DOTB), dim3(256), 0, c.stream, (const float *)x,
                            (const float *)v, (const float *)vt, c.R, c.Tau, c.Gamma, part);
-    hipLaunchKernelGGL(k_fv_fin, dim3(1), dim3(256), 0, c.stream, part, DOTB, (double)c.R, c.scal + SC_FV);
-    c.g->force(c, x, v, c.scal + SC_FV, coef);
+    hipLaunchKernelGGL(k_fv_fin<1>, dim3(1), dim3(256), 0, c.stream, part, DOTB, (double)c.R, nullptr,
+                       c.scal + SC_FV, nullptr);
     return spm_launch_status("forces");
-}
-
-int spring(spm_ctx *h, void *p0, void *p1, double dt, double ks)
-{
-    Ctx &c = h->c;
-    if (c.prec == SPM_FP64)
-        hipLaunchKernelGGL(k_spring<double>, dim3(2048), dim3(256), 0, c.stream, (double *)p0, (double *)p1, c.R,
-                           dt, ks);
-    else
-        hipLaunchKernelGGL(k_spring<float>, dim3(2048), dim3(256), 0, c.stream, (float *)p0, (float *)p1, c.R, dt,
-                           ks);
-    return spm_launch_status("spring");
 }
 
 int ensure_fields(spm_ctx *h, int n)
@@ -162,6 +162,18 @@ int ensure_fields(spm_ctx *h, int n)
     return SPM_OK;
 }
 
+// Natural_Forces (coef 1) / MinMax_Forces (coef 2) on x in place along the
+// unit tangent v: one G convolution of v, the <f, v> sums, one force round trip.
+int unit_force(spm_ctx *h, void *x, const void *v, double coef)
+{
+    Ctx &c = h->c;
+    RCHK(ensure_fields(h, 1));
+    c.g->gconv(c, v, h->ws);
+    RCHK(fv_scalar(h, x, v, h->ws));
+    c.g->force(c, x, v, c.scal + SC_FV, coef);
+    return spm_launch_status("forces");
+}
+
 } // namespace
 
 extern "C" {
@@ -170,55 +182,57 @@ int spm_get_tv(spm_ctx *h, const void *phi0, const void *phi1, void *v)
 {
     CHECK_CTX(h);
     if (!phi0 || !phi1 || !v) return spm_fail(SPM_EINVAL, "spm_get_tv: null field");
-    return get_tv(h, phi0, phi1, v);
+    return h->c.prec == SPM_FP64 ? launch_tv<double>(h, phi0, phi1, v) : launch_tv<float>(h, phi0, phi1, v);
 }
 
 int spm_natural_force(spm_ctx *h, void *x, const void *v)
 {
     CHECK_CTX(h);
     if (!x || !v || x == v) return spm_fail(SPM_EINVAL, "spm_natural_force: bad fields");
-    RCHK(ensure_fields(h, 1));
-    c.g->gconv(c, v, h->ws);
-    return forces(h, x, v, h->ws, 1.0);
+    return unit_force(h, x, v, 1.0);
 }
 
 int spm_minmax_force(spm_ctx *h, void *x, const void *v)
 {
     CHECK_CTX(h);
     if (!x || !v || x == v) return spm_fail(SPM_EINVAL, "spm_minmax_force: bad fields");
-    RCHK(ensure_fields(h, 1));
-    c.g->gconv(c, v, h->ws);
-    return forces(h, x, v, h->ws, 2.0);
+    return unit_force(h, x, v, 2.0);
 }
 
+// One iteration is, in the reference's order (SPM.cpp:183-254):
+//   get_Tv; Natural_Forces on both replicas; spring; get_Tv;
+//   MinMax_Forces and ngrad_cam on both replicas; spring.
+// Here the tangent d = Phi1 - Phi0 stays unnormalised: the force of replica r
+// along t = d / nrmp(d) is <f_r, t> t = (<f_r, d> / nrmp(d)^2) d, so the
+// division by nrmp(d) moves into the two force scalars.  A force needs
+// <f_r, d>, which takes G*d; one round trip of d gives both replicas' sums (the
+// forces on one replica do not touch the other), and its input pass also does
+// the spring step that precedes each get_Tv.  So the spring at the end of an
+// iteration also forms the next iteration's tangent.  ngrad_cam keeps only its
+// norm (the residual), and its input transform is the force round trip's
+// output pass.  The host reads the residual once per iteration.
 int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double tol, int *iters, double *res_out,
                        double *res_hist)
 {
     CHECK_CTX(h);
     if (!phi0 || !phi1 || phi0 == phi1) return spm_fail(SPM_EINVAL, "spm_spring_pairing: bad replicas");
     if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_spring_pairing: maxiter >= 0");
-    RCHK(ensure_fields(h, 3));
-    void *tv = h->ws;
-    void *f = (char *)h->ws + (size_t)c.R * c.esz;
-    void *vt = (char *)h->ws + 2 * (size_t)c.R * c.esz; // G * tv
+    RCHK(ensure_fields(h, 1));
+    void *d = h->ws; // Phi1 - Phi0
     void *Phi[2] = {phi0, phi1};
     const double dt = 0.001, ks = 1; // SPM.cpp:177-178
     double res = 1e-6;               // SPM.cpp:180
     int iter = 0;
+    if (res > tol && maxiter > 0) RCHK(tangent(h, Phi[0], Phi[1], d, false, dt, ks));
     while (res > tol && iter < maxiter) {
         iter = iter + 1;
-        RCHK(get_tv(h, Phi[0], Phi[1], tv));
-        c.g->gconv(c, tv, vt);
-        for (int i = 0; i < 2; ++i) RCHK(forces(h, Phi[i], tv, vt, 1.0));
-        RCHK(spring(h, Phi[0], Phi[1], dt, ks));
-        RCHK(get_tv(h, Phi[0], Phi[1], tv));
-        c.g->gconv(c, tv, vt);
+        for (int i = 0; i < 2; ++i) c.g->force(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 1.0);
+        RCHK(tangent(h, Phi[0], Phi[1], d, true, dt, ks));
         for (int i = 0; i < 2; ++i) {
-            RCHK(forces(h, Phi[i], tv, vt, 2.0));
-            c.g->ngrad(c, Phi[i], f);
+            c.g->force_ngrad(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 2.0);
             hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, c.stream, c.scal + SC_GN2, c.scal + SC_RES0 + i);
         }
-        RCHK(spring(h, Phi[0], Phi[1], dt, ks));
+        RCHK(tangent(h, Phi[0], Phi[1], d, true, dt, ks)); // and the next iteration's tangent
         double g2[2];
         RCHK(spm_read_scalars(h, c.scal + SC_RES0, 2, g2));
         res = 0.0;
@@ -228,7 +242,6 @@ int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double t
         }
         if (res_hist) res_hist[iter - 1] = res;
     }
-    RCHK(get_tv(h, Phi[0], Phi[1], tv));
     HIPCHK(hipStreamSynchronize(c.stream));
     if (iters) *iters = iter;
     if (res_out) *res_out = res;
