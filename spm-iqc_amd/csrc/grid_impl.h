@@ -228,26 +228,13 @@ struct GridImpl final : GridIface {
         rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)out}, nullptr);
     }
 
-    void force(Ctx &c, void *x, const void *v, const double *s, double coef) override
+    void force(Ctx &c, void *x, const void *v, const double *s, double coef, ForceSpec *fs) override
     {
-        if constexpr (N == 64 || N == 128 || N == 256) {
-            if (two_pass(c)) {
-                sj_round_trip(c, x, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr, [] {},
-                              SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, OpStore<T>{(T *)x}, nullptr);
-                return;
-            }
-        }
-        rows<false, true>(c, x, c.W, OpForceIn<T>{c.Tau, c.Gamma, coef, (const T *)v, s}, nullptr);
-        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
-        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Btab}, nullptr);
-        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
-        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+        force_any(c, x, v, s, coef, fs, false);
     }
-
-    long spm_tangent(Ctx &c, void *p0, void *p1, void *d, bool spring, double dt, double ks, double *pt,
-                     double *pf) override
+    long spm_tangent(Ctx &c, const void *p0, const void *p1, void *d, double *pt, double *pf) override
     {
-        const OpSpringIn<T> in{dt, ks, spring, (T *)p0, (T *)p1, (T *)d};
+        const OpTangentIn<T> in{(const T *)p1, (T *)d};
         const OpFvOut<T> out{c.Tau, c.Gamma, (const T *)p0, (const T *)p1, (const T *)d};
         const SOpMulTab<T, N> g{1.0 / (double)c.R, c.Gtab};
         if constexpr (N == 64 || N == 128 || N == 256) {
@@ -264,9 +251,16 @@ struct GridImpl final : GridIface {
         return c.nb_rows;
     }
 
-    // force, then ngrad_cam's norm at the updated x: the output pass of the
-    // force round trip is also the input pass of ngrad's (one pass over x fewer).
-    void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef) override
+    void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef, ForceSpec *fs) override
+    {
+        force_any(c, x, v, s, coef, fs, true);
+    }
+
+    // One SPM force step on x (and, with ngrad, ngrad_cam's norm at the result:
+    // the output pass of the force round trip is also the input pass of
+    // ngrad's, one pass over x fewer).  sop is the force's spectral step.
+    template <class SOp>
+    void force_with(Ctx &c, void *x, const void *v, const double *s, double coef, SOp sop, double *sp, bool ngrad)
     {
         const double sc = 1.0 / (double)c.R;
         const OpForceIn<T> fin{c.Tau, c.Gamma, coef, (const T *)v, s};
@@ -275,7 +269,11 @@ struct GridImpl final : GridIface {
         if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
                 sj_rows<false, true>(c, x, c.W, fin, nullptr, nullptr);
-                sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Btab}, nullptr, nullptr);
+                sj_cols<true, true>(c, c.W, c.W, sop, sp, nullptr);
+                if (!ngrad) {
+                    sj_rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr, nullptr);
+                    return;
+                }
                 sj_rows<true, true>(c, c.W, c.W, mid, c.slot(0), nullptr);
                 launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, sc, c.scal + SC_MEAN_F);
                 sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr);
@@ -286,8 +284,12 @@ struct GridImpl final : GridIface {
         }
         rows<false, true>(c, x, c.W, fin, nullptr);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
-        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Btab}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, sop, sp);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        if (!ngrad) {
+            rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+            return;
+        }
         rows<true, true>(c, c.W, c.W, mid, c.slot(0));
         launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, sc, c.scal + SC_MEAN_F);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
@@ -295,6 +297,20 @@ struct GridImpl final : GridIface {
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
         rows<true, false>(c, c.W, nullptr, nout, c.slot(1));
         launch_reduce(c, c.slot(1), c.nb_rows, 1, 0u, sc, c.scal + SC_GN2);
+    }
+
+    void force_any(Ctx &c, void *x, const void *v, const double *s, double coef, ForceSpec *fs, bool ngrad)
+    {
+        const double sc = 1.0 / (double)c.R;
+        if (!fs) return force_with(c, x, v, s, coef, SOpMulTab<T, N>{sc, c.Btab}, nullptr, ngrad);
+        if (!fs->other)
+            return force_with(c, x, v, s, coef, SOpForceSpec<T, N, false>{sc, c.Btab, c.Gtab, (CT *)fs->keep, nullptr},
+                              nullptr, ngrad);
+        fs->nb = ((long)N * c.NHP + GI::B - 1) / GI::B; // blocks of the i pass
+        if constexpr (N == 64 || N == 128 || N == 256)
+            if (two_pass(c)) fs->nb = SJT::C_BLOCKS;
+        force_with(c, x, v, s, coef, SOpForceSpec<T, N, true>{sc, c.Btab, c.Gtab, nullptr, (const CT *)fs->other},
+                   fs->part, ngrad);
     }
 
     // ---------------------------------------------------------- gradient flow

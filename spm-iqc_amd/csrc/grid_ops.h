@@ -182,29 +182,18 @@ template <typename T> struct OpStoreNgradIn {
     }
 };
 
-// Spring_Pairing's tangent (SPM.cpp:15-32, 203-218) as the input pass of its
-// G convolution.  x = Phi0 (the pass's input), Phi1 is gathered; with spring,
-// the spring half-steps of SPM.cpp:203-216 update both replicas in place
-// (Phi0 += dt ks (Phi1 - Phi0), then Phi1 += dt ks (Phi0' - Phi1)).  The
-// unnormalised tangent d = Phi1 - Phi0 is stored and transformed, and the sum
-// of d^2 accumulated (nrmp(d)^2).
-template <typename T> struct OpSpringIn {
+// Spring_Pairing's first tangent (get_Tv, SPM.cpp:15-32) as the input pass of
+// its G convolution: x = Phi0 (the pass's input), Phi1 gathered; the
+// unnormalised tangent d = Phi1 - Phi0 is stored and transformed, and the sum of
+// d^2 accumulated (nrmp(d)^2).
+template <typename T> struct OpTangentIn {
     static constexpr int NRED = 1;
     static constexpr unsigned MAXM = 0;
-    double dt, ks;
-    bool spring;
-    T *p0, *p1, *d;
+    const T *p1;
+    T *d;
     __device__ __forceinline__ T operator()(T x, long g, double *acc) const
     {
-        T a1 = x, b1 = p1[g];
-        if (spring) {
-            const double a = (double)a1, b = (double)b1;
-            a1 = (T)(1 * a + dt * (ks * b + -ks * a));
-            b1 = (T)(1 * b + dt * (ks * (double)a1 + -ks * b));
-            p0[g] = a1;
-            p1[g] = b1;
-        }
-        const T di = b1 - a1;
+        const T di = p1[g] - x;
         d[g] = di;
         acc[0] += (double)di * (double)di;
         return di;
@@ -354,6 +343,44 @@ template <typename T, int N> struct SOpMulTab {
 
 // Hermitian weight of a half-spectrum entry in a sum over the full spectrum.
 template <int N> __device__ __forceinline__ double hweight(int kz) { return (kz == 0 || kz == N / 2) ? 1.0 : 2.0; }
+
+// Spring_Pairing's force passes (Natural_Forces / MinMax_Forces, SPM.cpp:83-91):
+// v <- sc B v with B = 1/(1+G) (DC = 0), which is the normalised spectrum X of
+// the updated replica.  The first replica's pass stores X at the spectrum index
+// (keep); the second (CROSS) reads it back and sums, over k != 0 with the
+// Hermitian weights, Q_pq = sum w G Re(X_p conj X_q) for (p, q) = (0, 0), (0, 1),
+// (1, 1): the means <x_p, G*x_q> of the two updated replicas (Parseval).
+// FP64: the stored spectrum is loaded at the start of the pass (PRELOAD), so
+// its latency overlaps the forward transform.  (FP32 pass-2 blocks of 512
+// threads would then fit once per CU; they gather it where it is used.)
+template <typename T, int N, bool CROSS> struct SOpForceSpec {
+    static constexpr int NRED = CROSS ? 3 : 0;
+    static constexpr unsigned MAXM = 0;
+    static constexpr bool PRELOAD = CROSS && sizeof(T) == 8, FINALIZE = false;
+    double sc;
+    const double *Btab, *Gtab;
+    C2<T> *keep;         // !CROSS
+    const C2<T> *other;  // CROSS
+    __device__ __forceinline__ C2<T> load_pre(long s) const { return other[s]; }
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T> o, long s, int i, int j, int kz, double *acc) const
+    {
+        const int a = kfold(i, N), b = kfold(j, N);
+        const int n2 = a * a + b * b + kz * kz;
+        const double f = n2 == 0 ? 0.0 : sc * Btab[n2];
+        v.x = (T)((double)v.x * f);
+        v.y = (T)((double)v.y * f);
+        if constexpr (CROSS) {
+            if constexpr (!PRELOAD) o = other[s];
+            const double wg = n2 == 0 ? 0.0 : hweight<N>(kz) * Gtab[n2];
+            const double ox = (double)o.x, oy = (double)o.y, vx = (double)v.x, vy = (double)v.y;
+            acc[0] += wg * (ox * ox + oy * oy);
+            acc[1] += wg * (ox * vx + oy * vy);
+            acc[2] += wg * (vx * vx + vy * vy);
+        } else {
+            keep[s] = v;
+        }
+    }
+};
 
 // Non-local energy sum of get_Ene (LPsrc.cpp:27-35): acc += w G |sc v|^2.
 template <typename T, int N> struct SOpEnergy {

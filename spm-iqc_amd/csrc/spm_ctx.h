@@ -60,6 +60,16 @@ struct Ctx {
 };
 
 // Transform and operator passes specialised for (precision, N).
+// Spring_Pairing's force pair (SOpForceSpec): the first replica's force stores
+// the spectrum of its result in keep; the second reads it (other) and writes the
+// partial sums of Q_00, Q_01, Q_11 into part, nb blocks of 3 (set by the call).
+struct ForceSpec {
+    void *keep = nullptr;
+    const void *other = nullptr;
+    double *part = nullptr;
+    long nb = 0;
+};
+
 struct GridIface {
     virtual ~GridIface() = default;
     virtual void geometry(long &nb_rows, long &nb_j, long &nb_i, int NHP) = 0;
@@ -81,14 +91,14 @@ struct GridIface {
     virtual void gconv(Ctx &c, const void *in, void *out) = 0;
     // x <- IFFT(FFT(x + nln(x) - coef * (*s) * v) / N^3 / (1 + G)), DC = 0
     // (one force step of SPM.cpp:34-99 / 103-168; s a device scalar)
-    virtual void force(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;
-    // Spring_Pairing's tangent d = Phi1 - Phi0 (after the spring half-steps on
-    // p0, p1 when spring): partial sums of d^2 into pt (one per block), the <f, d>
-    // sums of both replicas (OpFvOut, 7 per block) into pf; returns the block count
-    virtual long spm_tangent(Ctx &c, void *p0, void *p1, void *d, bool spring, double dt, double ks, double *pt,
-                             double *pf) = 0;
+    virtual void force(Ctx &c, void *x, const void *v, const double *s, double coef, ForceSpec *fs = nullptr) = 0;
+    // Spring_Pairing's first tangent d = Phi1 - Phi0: partial sums of d^2 into pt
+    // (one per block), the <f, d> sums of both replicas (OpFvOut, 7 per block)
+    // into pf; returns the block count
+    virtual long spm_tangent(Ctx &c, const void *p0, const void *p1, void *d, double *pt, double *pf) = 0;
     // force, then scal[SC_GN2] = nrmp(ngrad_cam(x))^2 of the updated x (no gradient field)
-    virtual void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef) = 0;
+    virtual void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef,
+                             ForceSpec *fs = nullptr) = 0;
     virtual void gf_init(Ctx &c, const void *phi) = 0;
     virtual void gf_init_member(Ctx &c, int m, const void *phi, double Tau, double Gamma) = 0;
     virtual void gf_read_member(Ctx &c, int m, void *phi) = 0;

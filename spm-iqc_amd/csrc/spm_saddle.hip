@@ -3,6 +3,7 @@
 // on two device-resident replicas.  Scalars that the reference gathers with
 // MPI_Allreduce (the tangent norm, <f, v>) stay on the device; the host reads
 // only the residual, once per iteration.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -100,6 +101,79 @@ template <typename T> __global__ void k_div_by_sqrt(T *v, long n, const double *
         v[i] = (T)((double)v[i] / d);
 }
 
+// The spring half-steps of SPM.cpp:203-216 (and 233-246), replicas updated in
+// the reference's order: Phi0 += dt ks (Phi1 - Phi0); Phi1 += dt ks (Phi0' - Phi1).
+// The get_Tv that follows (SPM.cpp:218, and 185 of the next iteration) rides
+// along: d = Phi1' - Phi0', left unnormalised.  So do the real-space sums of the
+// next two force scalars: d^2; per replica nl_r d and nl_r
+// (nl = Tau x + Gamma x^2 - x^3); d.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spring_sums(T *p0, T *p1, T *__restrict__ d, long n, double dt, double ks,
+                                                    double Tau, double Gamma, double *__restrict__ part)
+{
+    __shared__ double red[6 * 16];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double a = (double)p0[i], b = (double)p1[i];
+        const T a1 = (T)(1 * a + dt * (ks * b + -ks * a));
+        const T b1 = (T)(1 * b + dt * (ks * (double)a1 + -ks * b));
+        p0[i] = a1;
+        p1[i] = b1;
+        const T di = b1 - a1;
+        d[i] = di;
+        const double dd = (double)di, u0 = (double)a1, u1 = (double)b1;
+        const double n0 = Tau * u0 + Gamma * u0 * u0 - u0 * u0 * u0;
+        const double n1 = Tau * u1 + Gamma * u1 * u1 - u1 * u1 * u1;
+        s[0] += dd * dd;
+        s[1] += n0 * dd;
+        s[2] += n0;
+        s[3] += n1 * dd;
+        s[4] += n1;
+        s[5] += dd;
+    }
+    block_reduce<6, 0u>(s, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) part[(long)q * gridDim.x + blockIdx.x] = s[q];
+}
+
+// The force scalars after a spring step: tv2 = nrmp(d)^2 and, for both
+// replicas, <f_r, d> / nrmp(d)^2 (k_fv_fin's formula).  Its G-convolution terms
+// <x_r, G*d> come from the quadratic forms Q_pq = <x_p, G*x_q> of the replicas
+// before the spring step (pq: sums of the force passes, SOpForceSpec), because
+// that step is linear: with a = dt ks,
+//   x0' = (1 - a) x0 + a x1,  x1' = a (1 - a) x0 + (1 - a + a^2) x1,
+//   d' = x1' - x0' = (1 - a)^2 (x1 - x0).
+__global__ void __launch_bounds__(256) k_spring_fin(const double *__restrict__ ps, int nbs,
+                                                   const double *__restrict__ pq, int nbq, double R, double a,
+                                                   double *tv2, double *out0, double *out1)
+{
+    __shared__ double red[9 * 16];
+    double s[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) s[q] = 0.0;
+    for (int b = threadIdx.x; b < nbs; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s[q] += ps[(long)q * nbs + b];
+    for (int b = threadIdx.x; b < nbq; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s[6 + q] += pq[(long)q * nbq + b];
+    block_reduce<9, 0u>(s, red);
+    if (threadIdx.x == 0) {
+        const double Q00 = s[6], Q01 = s[7], Q11 = s[8];
+        const double A[2][2] = {{1 - a, a}, {a * (1 - a), 1 - a + a * a}};
+        const double D0 = -(1 - a) * (1 - a), D1 = (1 - a) * (1 - a);
+        const double d2 = s[0] / R, mv = s[5] / R;
+        *tv2 = d2;
+        double *out[2] = {out0, out1};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const double xgd = A[r][0] * (D0 * Q00 + D1 * Q01) + A[r][1] * (D0 * Q01 + D1 * Q11);
+            *out[r] = (-xgd + s[1 + 2 * r] / R - (s[2 + 2 * r] / R) * mv) / d2;
+        }
+    }
+}
+
 __global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
 
 // get_Tv: v = (Phi1 - Phi0) / nrmp(Phi1 - Phi0)
@@ -114,20 +188,39 @@ template <typename T> int launch_tv(spm_ctx *h, const void *p0, const void *p1, 
     return spm_launch_status("get_Tv");
 }
 
-// Spring_Pairing's tangent (after the spring half-steps when spring): d = Phi1 -
-// Phi0 unnormalised, scal[SC_TV2] = nrmp(d)^2 and the force scalars
-// scal[SC_FV], scal[SC_FV1] = <f_r, d> / nrmp(d)^2 of both replicas, from one
-// G-convolution round trip of d (its passes carry the spring step and the sums).
-int tangent(spm_ctx *h, void *p0, void *p1, void *d, bool spring, double dt, double ks)
+// Spring_Pairing's first tangent: d = Phi1 - Phi0 unnormalised,
+// scal[SC_TV2] = nrmp(d)^2 and the force scalars scal[SC_FV], scal[SC_FV1] =
+// <f_r, d> / nrmp(d)^2 of both replicas, from one G-convolution round trip of d
+// (its passes form d and the sums).
+int first_tangent(spm_ctx *h, const void *p0, const void *p1, void *d)
 {
     Ctx &c = h->c;
     double *pt = c.slot(5), *pf = c.slot(6);
-    const long nb = c.g->spm_tangent(c, p0, p1, d, spring, dt, ks, pt, pf);
+    const long nb = c.g->spm_tangent(c, p0, p1, d, pt, pf);
     if (7 * nb > c.part_slot) return spm_fail(SPM_EINVAL, "spm tangent: partial sums exceed a slot");
     launch_reduce(c, pt, nb, 1, 0u, 1.0 / (double)c.R, c.scal + SC_TV2);
     hipLaunchKernelGGL(k_fv_fin<2>, dim3(1), dim3(256), 0, c.stream, pf, (int)nb, (double)c.R, c.scal + SC_TV2,
                        c.scal + SC_FV, c.scal + SC_FV1);
     return spm_launch_status("spm tangent");
+}
+
+// The spring step of a replica pair and the next tangent d with its force
+// scalars; fs holds the Q sums of the force pair just done.
+int spring_tangent(spm_ctx *h, void *p0, void *p1, void *d, double dt, double ks, const ForceSpec &fs)
+{
+    Ctx &c = h->c;
+    double *ps = c.slot(5);
+    // 2048 blocks (8 per CU) keep enough loads in flight; 6 partial sums per block
+    const int nbs = (int)std::min<long>(2048, c.part_slot / 6);
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_spring_sums<double>, dim3(nbs), dim3(256), 0, c.stream, (double *)p0, (double *)p1,
+                           (double *)d, c.R, dt, ks, c.Tau, c.Gamma, ps);
+    else
+        hipLaunchKernelGGL(k_spring_sums<float>, dim3(nbs), dim3(256), 0, c.stream, (float *)p0, (float *)p1,
+                           (float *)d, c.R, dt, ks, c.Tau, c.Gamma, ps);
+    hipLaunchKernelGGL(k_spring_fin, dim3(1), dim3(256), 0, c.stream, ps, nbs, fs.part, (int)fs.nb, (double)c.R,
+                       dt * ks, c.scal + SC_TV2, c.scal + SC_FV, c.scal + SC_FV1);
+    return spm_launch_status("spring");
 }
 
 // <f, v> of x along the unit tangent v into scal[SC_FV], given vt = G*v.
@@ -146,9 +239,8 @@ int fv_scalar(spm_ctx *h, const void *x, const void *v, const void *vt)
     return spm_launch_status("forces");
 }
 
-int ensure_fields(spm_ctx *h, int n)
+int ensure_ws(spm_ctx *h, size_t bytes)
 {
-    const size_t bytes = (size_t)n * h->c.R * h->c.esz;
     if (h->ws_bytes >= bytes) return SPM_OK;
     if (h->ws) {
         HIPCHK(hipStreamSynchronize(h->c.stream));
@@ -167,7 +259,7 @@ int ensure_fields(spm_ctx *h, int n)
 int unit_force(spm_ctx *h, void *x, const void *v, double coef)
 {
     Ctx &c = h->c;
-    RCHK(ensure_fields(h, 1));
+    RCHK(ensure_ws(h, (size_t)c.R * c.esz));
     c.g->gconv(c, v, h->ws);
     RCHK(fv_scalar(h, x, v, h->ws));
     c.g->force(c, x, v, c.scal + SC_FV, coef);
@@ -204,35 +296,43 @@ int spm_minmax_force(spm_ctx *h, void *x, const void *v)
 //   MinMax_Forces and ngrad_cam on both replicas; spring.
 // Here the tangent d = Phi1 - Phi0 stays unnormalised: the force of replica r
 // along t = d / nrmp(d) is <f_r, t> t = (<f_r, d> / nrmp(d)^2) d, so the
-// division by nrmp(d) moves into the two force scalars.  A force needs
-// <f_r, d>, which takes G*d; one round trip of d gives both replicas' sums (the
-// forces on one replica do not touch the other), and its input pass also does
-// the spring step that precedes each get_Tv.  So the spring at the end of an
-// iteration also forms the next iteration's tangent.  ngrad_cam keeps only its
-// norm (the residual), and its input transform is the force round trip's
-// output pass.  The host reads the residual once per iteration.
+// division by nrmp(d) moves into the two force scalars.  <f_r, d> needs
+// <x_r, G*d>.  Every get_Tv but the first follows a spring step, which is
+// linear, so those terms are combinations of <x_p, G*x_q> of the replicas the
+// force pair has just produced; the force passes sum these in spectral space
+// (Parseval) from the spectra they already hold, and the spring kernel forms d
+// and the real-space sums.  So no transform of d is needed after the first.
+// ngrad_cam keeps only its norm (the residual), and its input transform is the
+// force round trip's output pass.  The host reads the residual once per iteration.
 int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double tol, int *iters, double *res_out,
                        double *res_hist)
 {
     CHECK_CTX(h);
     if (!phi0 || !phi1 || phi0 == phi1) return spm_fail(SPM_EINVAL, "spm_spring_pairing: bad replicas");
     if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_spring_pairing: maxiter >= 0");
-    RCHK(ensure_fields(h, 1));
-    void *d = h->ws; // Phi1 - Phi0
+    const size_t dbytes = (size_t)c.R * c.esz;
+    RCHK(ensure_ws(h, dbytes + (size_t)c.C * 2 * c.esz));
+    void *d = h->ws;                    // Phi1 - Phi0
+    void *keep = (char *)h->ws + dbytes; // spectrum of the first replica after its force
     void *Phi[2] = {phi0, phi1};
     const double dt = 0.001, ks = 1; // SPM.cpp:177-178
     double res = 1e-6;               // SPM.cpp:180
     int iter = 0;
-    if (res > tol && maxiter > 0) RCHK(tangent(h, Phi[0], Phi[1], d, false, dt, ks));
+    ForceSpec f0, f1;
+    f0.keep = keep;
+    f1.other = keep;
+    f1.part = c.slot(7);
+    if (res > tol && maxiter > 0) RCHK(first_tangent(h, Phi[0], Phi[1], d));
     while (res > tol && iter < maxiter) {
         iter = iter + 1;
-        for (int i = 0; i < 2; ++i) c.g->force(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 1.0);
-        RCHK(tangent(h, Phi[0], Phi[1], d, true, dt, ks));
+        c.g->force(c, Phi[0], d, c.scal + SC_FV, 1.0, &f0);
+        c.g->force(c, Phi[1], d, c.scal + SC_FV1, 1.0, &f1);
+        RCHK(spring_tangent(h, Phi[0], Phi[1], d, dt, ks, f1));
         for (int i = 0; i < 2; ++i) {
-            c.g->force_ngrad(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 2.0);
+            c.g->force_ngrad(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 2.0, i ? &f1 : &f0);
             hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, c.stream, c.scal + SC_GN2, c.scal + SC_RES0 + i);
         }
-        RCHK(tangent(h, Phi[0], Phi[1], d, true, dt, ks)); // and the next iteration's tangent
+        RCHK(spring_tangent(h, Phi[0], Phi[1], d, dt, ks, f1)); // and the next iteration's tangent
         double g2[2];
         RCHK(spm_read_scalars(h, c.scal + SC_RES0, 2, g2));
         res = 0.0;
