@@ -210,6 +210,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     T *my = lds + pp * RSTR;
     C2<T> v[E];
     C2<T> c[DH];
+    C2<T> aux[!IN_SPEC && has_aux<Op>::value ? E : 1]; // the op's other field, loaded with the input
     double acc[Op::NRED > 0 ? Op::NRED : 1];
 #pragma unroll
     for (int q = 0; q < (Op::NRED > 0 ? Op::NRED : 1); ++q) acc[q] = 0.0;
@@ -286,6 +287,12 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         const T *xa = X + ((long)i * N + ja) * N, *xb = X + ((long)i * N + jb) * N;
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = C2<T>{xa[t + TPP * m], xb[t + TPP * m]};
+        if constexpr (has_aux<Op>::value) {
+            const T *A = op.aux();
+            const T *aa = A + ((long)i * N + ja) * N, *ab = A + ((long)i * N + jb) * N;
+#pragma unroll
+            for (int m = 0; m < E; ++m) aux[m] = C2<T>{aa[t + TPP * m], ab[t + TPP * m]};
+        }
         __syncthreads(); // twiddles
     }
     // The pointwise work sits in a basic block of its own (the condition always
@@ -302,8 +309,13 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
 #pragma unroll
                 for (int m = m0; m < m0 + CH; ++m) {
                     const int p = t + TPP * m;
-                    v[m].x = op(v[m].x, ga + p, acc);
-                    v[m].y = op(v[m].y, gb + p, acc);
+                    if constexpr (!IN_SPEC && has_aux<Op>::value) {
+                        v[m].x = op(v[m].x, aux[m].x, ga + p, acc);
+                        v[m].y = op(v[m].y, aux[m].y, gb + p, acc);
+                    } else {
+                        v[m].x = op(v[m].x, ga + p, acc);
+                        v[m].y = op(v[m].y, gb + p, acc);
+                    }
                 }
             }
         }

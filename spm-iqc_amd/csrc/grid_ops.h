@@ -80,6 +80,13 @@ template <class Op> __device__ __forceinline__ Op bind_member(const Op &op, int 
 template <class Op, class = void> struct op_chunk : std::integral_constant<int, 0> {};
 template <class Op> struct op_chunk<Op, std::void_t<decltype(Op::CHUNK)>> : std::integral_constant<int, Op::CHUNK> {};
 
+// An input-pass operator that reads one other field at the same index may
+// declare it as aux() and take its value as an argument, op(x, a, g, acc): the
+// two-pass row kernel then loads it together with its input, at the start, so
+// the two loads overlap.  op(x, g, acc) gathers it itself (the axis-by-axis passes).
+template <class Op, class = void> struct has_aux : std::false_type {};
+template <class Op> struct has_aux<Op, std::void_t<decltype(std::declval<const Op &>().aux())>> : std::true_type {};
+
 // ======================================================================
 // Real-space operators applied to each real value inside the row pass.
 // op(x, g, acc): x = value at real-field index g; returns the value the pass
@@ -191,13 +198,15 @@ template <typename T> struct OpTangentIn {
     static constexpr unsigned MAXM = 0;
     const T *p1;
     T *d;
-    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    __device__ __forceinline__ const T *aux() const { return p1; }
+    __device__ __forceinline__ T operator()(T x, T b, long g, double *acc) const
     {
-        const T di = p1[g] - x;
+        const T di = b - x;
         d[g] = di;
         acc[0] += (double)di * (double)di;
         return di;
     }
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, p1[g], g, acc); }
 };
 
 // The output pass of that G convolution: with w = (G*d)(g), the sums for
@@ -254,12 +263,14 @@ template <typename T> struct OpHvIn {
     static constexpr unsigned MAXM = 0;
     double Tau, Gamma;
     const T *phi;
-    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
+    __device__ __forceinline__ const T *aux() const { return phi; }
+    __device__ __forceinline__ T operator()(T x, T pg, long, double *acc) const
     {
-        const double u = (double)phi[g];
+        const double u = (double)pg;
         acc[0] += (-Tau - 2 * Gamma * u + 3 * u * u) * (double)x;
         return x;
     }
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, phi[g], g, acc); }
 };
 
 // hv_cam epilogue (LOBPCG.cpp:136-156): hv = x + c(phi) v - mean(c v), with x
@@ -289,11 +300,13 @@ template <typename T> struct OpForceIn {
     double Tau, Gamma, coef;
     const T *v;
     const double *s;
-    __device__ __forceinline__ T operator()(T x, long g, double *) const
+    __device__ __forceinline__ const T *aux() const { return v; }
+    __device__ __forceinline__ T operator()(T x, T vg, long, double *) const
     {
         const double u = (double)x, u2 = u * u, u3 = u2 * u;
-        return (T)(u + (Tau * u + Gamma * u2 - u3) - coef * (*s) * (double)v[g]);
+        return (T)(u + (Tau * u + Gamma * u2 - u3) - coef * (*s) * (double)vg);
     }
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, v[g], g, acc); }
 };
 
 // ======================================================================
