@@ -73,10 +73,11 @@ template <typename T, int N> struct SJ {
         kg = s;
         return P < DH * KO;
     }
-    // U: the state of one pass-2 block is one contiguous chunk [i][l][ka]
-    __host__ __device__ static constexpr long uof(int h, int ko, int kg, int i, int c)
+    // U: the state of one pass-2 block is one contiguous chunk [l][i][ka]
+    // (digit k2 at l); r = i KB + ka is the pass-2 thread of the line role
+    __host__ __device__ static constexpr long uof(int h, int ko, int kg, int l, int r)
     {
-        return ((((long)h * KO + ko) * NSIB + kg) * N + i) * NC + c;
+        return ((((long)h * KO + ko) * NSIB + kg) * DL + l) * ((long)N * KB) + r;
     }
     static_assert(R_THREADS == S1 * (N / 2), "pass 1: one (l, kz < N/2) column per thread");
     static_assert(C_THREADS == N * KB, "pass 2: one (i, ka) line per thread");
@@ -377,12 +378,15 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
 
 // ----------------------------------------------------------------------
 // Pass 2: one (h, ko, kg): the columns (l, ka), ki = KB kg + ka, over all i.
-//   FWD: B (16-point DFTs over l) on W, then forward FFTs along i
+//   FWD: forward FFTs along i, then B (16-point DFTs over l) on W
 //   op:  the spectral operator at frequency (i, J = h + DH l, kz); PRELOAD
 //        operators read their state at the U offset
-//   INV: inverse FFTs along i, then B^-1, into W
+//   INV: B^-1, then inverse FFTs along i, into W
 // Without FWD the input is read from the U layout, without INV the output is
-// written to it.
+// written to it.  The i transforms run in the column role, whose W accesses
+// cover 4 x 512 bytes per wave (64 B of each 128-B line, the other half being
+// the sibling block's); B and the operator run in the line role, whose U
+// accesses are contiguous.  (B and the i transform commute.)
 // ----------------------------------------------------------------------
 template <typename T, int N, bool FWD, bool INV, class SOp, class Pro = ProNone>
 __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
@@ -392,6 +396,7 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
     using S = SJ<T, N>;
     constexpr int TPP = S::CTPP, E = N / TPP, KT = S::KT, KB = S::KB, NC = S::NC, DL = S::DL, DH = S::DH;
     constexpr int LS = NC + KB; // LDS stride of i: the line-role accesses then cover all banks
+    constexpr long LSTR = (long)N * KB; // U stride of l
     __shared__ T lds[N * LS];
     __shared__ C2<T> stw[twiddle_slots<N>()];
     __shared__ double red[(SOp::NRED > 0 ? SOp::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
@@ -407,69 +412,31 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
     load_twiddles<T, N>(stw, tw);
     int h, ko, kg;
     const bool real_blk = S::cblock(blockIdx.x, h, ko, kg);
-    // line role: thread (ia, ka) holds the DL values l of (i = ia, ki = KB kg + ka)
-    const int ka = threadIdx.x % KB, ia = threadIdx.x / KB;
+    // line role: thread r = (ia, ka) holds the DL values l of (i = ia, ki = KB kg + ka)
+    const int r = threadIdx.x, ka = r % KB, ia = r / KB;
     const int kia = KB * kg + ka;
     const bool va = real_blk && ko * KT + kia < S::NH;
-    const long lbase = S::wof(ia, h, ko, 0, kia);
+    const long ub = S::uof(h, ko, kg, 0, r);
     // column role: thread (cb, tb) holds column cb = (l, ka) at i = tb + TPP m
     const int cb = threadIdx.x % NC, tb = threadIdx.x / NC;
-    const int lb = cb / KB, kzb = ko * KT + KB * kg + cb % KB;
+    const int kzb = ko * KT + KB * kg + cb % KB;
     const bool vb = real_blk && kzb < S::NH;
-    const long ub = S::uof(h, ko, kg, 0, 0);
+    const long wbase = S::wof(0, h, ko, cb / KB, KB * kg + cb % KB);
     C2<T> v[E];
     static_assert(E == DL, "one register array serves both roles");
-    C2<T> pre[SOp::PRELOAD ? E : 1]; // state, loaded at the start: its latency overlaps the transforms
+    C2<T> pre[SOp::PRELOAD ? DL : 1]; // state, loaded at the start: its latency overlaps the transforms
     double acc[SOp::NRED > 0 ? SOp::NRED : 1];
 #pragma unroll
     for (int q = 0; q < (SOp::NRED > 0 ? SOp::NRED : 1); ++q) acc[q] = 0.0;
     if constexpr (FWD) {
 #pragma unroll
-        for (int q = 0; q < DL; ++q) v[q] = va ? src[lbase + q * KT] : C2<T>{0, 0};
+        for (int m = 0; m < E; ++m) v[m] = vb ? src[wbase + (long)(tb + TPP * m) * S::ISTR] : C2<T>{0, 0};
         if constexpr (SOp::PRELOAD) {
 #pragma unroll
-            for (int m = 0; m < E; ++m) pre[m] = vb ? op.load_pre(ub + (tb + TPP * m) * NC + cb) : C2<T>{0, 0};
+            for (int q = 0; q < DL; ++q) pre[q] = va ? op.load_pre(ub + q * LSTR) : C2<T>{0, 0};
         }
-        dft<16, -1>(v);
-        // line role -> column role: real parts, then imaginary parts
-        T hold[E];
-#pragma unroll
-        for (int q = 0; q < DL; ++q) {
-            lds[ia * LS + q * KB + ka] = v[q].x;
-            hold[q] = v[q].y;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m].x = lds[(tb + TPP * m) * LS + cb];
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < DL; ++q) lds[ia * LS + q * KB + ka] = hold[q];
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m].y = lds[(tb + TPP * m) * LS + cb];
         fft_pencil_split<T, N, TPP, -1, LS>(v, lds + cb, stw, tb);
-    } else {
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = vb ? src[ub + (tb + TPP * m) * NC + cb] : C2<T>{0, 0};
-    }
-    // The update in basic blocks of 4 elements (the conditions always hold):
-    // scheduled in one region, with its table gathers hoisted, it took far
-    // more registers.
-    if (vb) {
-        const int J = h + DH * lb;
-#pragma unroll
-        for (int m0 = 0; m0 < E; m0 += 4) {
-            if (blockIdx.x < gridDim.x + m0) {
-#pragma unroll
-                for (int m = m0; m < m0 + 4; ++m) {
-                    const int ii = tb + TPP * m;
-                    op(v[m], pre[SOp::PRELOAD ? m : 0], ub + ii * NC + cb, ii, J, kzb, acc);
-                }
-            }
-        }
-    }
-    if constexpr (INV) {
-        fft_pencil_split<T, N, TPP, +1, LS>(v, lds + cb, stw, tb);
+        // column role -> line role: real parts, then imaginary parts
         T hold[E];
         __syncthreads();
 #pragma unroll
@@ -486,14 +453,51 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < DL; ++q) v[q].y = lds[ia * LS + q * KB + ka];
-        dft<16, +1>(v);
-        if (va)
-#pragma unroll
-            for (int q = 0; q < DL; ++q) dst[lbase + q * KT] = v[q];
+        dft<16, -1>(v);
     } else {
+#pragma unroll
+        for (int q = 0; q < DL; ++q) v[q] = va ? src[ub + q * LSTR] : C2<T>{0, 0};
+    }
+    // The update in basic blocks of 4 elements (the conditions always hold):
+    // scheduled in one region, with its table gathers hoisted, it took far
+    // more registers.
+    if (va) {
+        const int kz = ko * KT + kia;
+#pragma unroll
+        for (int q0 = 0; q0 < DL; q0 += 4) {
+            if (blockIdx.x < gridDim.x + q0) {
+#pragma unroll
+                for (int q = q0; q < q0 + 4; ++q) op(v[q], pre[SOp::PRELOAD ? q : 0], ub + q * LSTR, ia, h + DH * q, kz, acc);
+            }
+        }
+    }
+    if constexpr (INV) {
+        dft<16, +1>(v);
+        // line role -> column role
+        T hold[E];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DL; ++q) {
+            lds[ia * LS + q * KB + ka] = v[q].x;
+            hold[q] = v[q].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m].x = lds[(tb + TPP * m) * LS + cb];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DL; ++q) lds[ia * LS + q * KB + ka] = hold[q];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m].y = lds[(tb + TPP * m) * LS + cb];
+        fft_pencil_split<T, N, TPP, +1, LS>(v, lds + cb, stw, tb);
         if (vb)
 #pragma unroll
-            for (int m = 0; m < E; ++m) dst[ub + (tb + TPP * m) * NC + cb] = v[m];
+            for (int m = 0; m < E; ++m) dst[wbase + (long)(tb + TPP * m) * S::ISTR] = v[m];
+    } else {
+        if (va)
+#pragma unroll
+            for (int q = 0; q < DL; ++q) dst[ub + q * LSTR] = v[q];
     }
     write_partials<SOp::NRED, SOp::MAXM>(acc, red, part);
 }
