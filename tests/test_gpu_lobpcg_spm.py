@@ -147,9 +147,11 @@ def test_lobpcg_matches_oracle_fixed_iterations(spm, po, seeds, torch):
     assert s.min() > 1 - 1e-8  # principal angles ~ 0
 
 
-def test_spm_forces_match_oracle(spm, po, seeds, torch):
-    """get_Tv, Natural_Forces, MinMax_Forces (SPM.cpp:15-168)."""
-    N = 32
+@pytest.mark.parametrize("N", [32, 64])
+def test_spm_forces_match_oracle(spm, po, seeds, torch, N):
+    """get_Tv, Natural_Forces, MinMax_Forces (SPM.cpp:15-168), by the axis-by-axis
+    passes (32^3) and the two-pass ones (64^3)."""
+    print("[case] spm forces vs oracle N=%d" % N, flush=True)
     p0 = seeds.icosahedral_nucleus(N, seed=1) if N >= 64 else seeds.gaussian(N, 0.05, 7)
     p1 = p0 + 0.01 * seeds.gaussian(N, 1.0, 8)
     o = po.Oracle(N)
@@ -170,9 +172,12 @@ def test_spm_forces_match_oracle(spm, po, seeds, torch):
         assert np.abs(_np(b) - x_mm.real).max() < 1e-12
 
 
-def test_spring_pairing_matches_oracle(spm, po, seeds, torch):
-    """Spring_Pairing (SPM.cpp:170-263): replicas and residuals per iteration."""
-    N, iters = 16, 12
+@pytest.mark.parametrize("N", [16, 64])
+def test_spring_pairing_matches_oracle(spm, po, seeds, torch, N):
+    """Spring_Pairing (SPM.cpp:170-263): replicas and residuals per iteration, by the
+    axis-by-axis passes (16^3) and the two-pass ones (64^3)."""
+    iters = 12
+    print("[case] spring_pairing vs oracle N=%d" % N, flush=True)
     p0 = _relaxed(po, seeds, N, 50, 3)
     p1 = p0 + 0.001 * seeds.gaussian(N, 1.0, 9)
     o = po.Oracle(N)
