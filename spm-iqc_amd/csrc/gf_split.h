@@ -10,15 +10,17 @@
 // Stage A rides in the row (kz) pass and stage B in the i pass, so one
 // iteration of get_min_u_F (LPsrc.cpp:52-111) is
 //   pass 1 (rows):    A^-1, c2r, f(phi) and the density sum, r2c, A     W in/out
-//   pass 2 (columns): B, i FFT, update with U and its sums, i FFT^-1, B^-1
+//   pass 2 (columns): i FFT, B, update with U and its sums, B^-1, i FFT^-1
 //                                                          W in/out, U in/out
+// (B and the i transform commute)
 // six array passes per step instead of the ten of the axis-by-axis passes.
 // The same passes make the transform round trips of the other operators.
 //
 // W of one field is stored as [i][h][ko][l][ki], kz = KT ko + ki, with
 // KT = 2 (FP64) or 4 (FP32).  A pass-1 block reads, per (h, ko), the 64-byte
 // half of a 128-byte line (2 l x KT ki); a pass-2 block reads, per i, every
-// other 16 bytes of 512 (16 l x KT/2 ki).  The two blocks sharing those lines
+// other 16 bytes of 512 (16 l x KT/2 ki), four i per wave instruction.  U is
+// chunked per pass-2 block as [l][i][ka].  The two blocks sharing W lines
 // are launched 8 block indices apart, i.e. on the same XCD at the same time,
 // so each line comes from HBM once.  Entries with kz > N/2 are padding.
 #pragma once
