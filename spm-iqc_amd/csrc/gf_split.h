@@ -222,9 +222,9 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         const C2<T> *W = (const C2<T> *)in + mb * in_bs;
 #pragma unroll
         for (int h = 0; h < DH; ++h) c[h] = W[S::wof(i, h, ko, l, ki)];
-        C2<T> ny = {0, 0};
-        if (tid < DH * S1) ny = W[S::wof(i, tid / S1, (N / 2) / KT, S1 * lo + tid % S1, (N / 2) % KT)];
-        __syncthreads(); // twiddles
+        // the kz = N/2 column input goes straight to LDS (no register across the A^-1 below)
+        if (tid < DH * S1) nyin[tid] = W[S::wof(i, tid / S1, (N / 2) / KT, S1 * lo + tid % S1, (N / 2) % KT)];
+        __syncthreads(); // twiddles, nyin
         // A^-1: conjugate twiddle, inverse 16-point DFT over h
 #pragma unroll
         for (int h = 1; h < DH; ++h) {
@@ -240,7 +240,6 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
             lds[(h * S1 + li) * RS + kz] = c[h].x;
             hold[h] = c[h].y;
         }
-        if (tid < DH * S1) nyin[tid] = ny;
         __syncthreads();
         if (tid < S1) { // the kz = N/2 columns: only their real parts enter the c2r
             const int ln = S1 * lo + tid;
@@ -325,6 +324,20 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     }
     if constexpr (OUT_SPEC) {
         fft_pencil_split<T, N, TPP, -1, 1, true>(v, my, stw, t);
+        // The indices of this phase again, from copies of the thread and block
+        // index that the compiler cannot match with the ones above: otherwise it
+        // keeps the addresses computed at the start live to here, and spills them.
+        int tid = threadIdx.x, bx = blockIdx.x;
+        asm volatile("" : "+v"(tid));
+        asm volatile("" : "+s"(bx));
+        int i, lo;
+        S::rblock(bx, i, lo);
+        const int li = LMAJ ? tid / (N / 2) : (tid / KT) % S1;
+        const int kzc = LMAJ ? tid % (N / 2) : (tid / (KT * S1)) * KT + tid % KT;
+        const int ki = kzc % KT, ko = kzc / KT;
+        const int kz = ko * KT + ki, l = S1 * lo + li;
+        const int t = tid % TPP;
+        T *my = lds + (tid / TPP) * RSTR;
         // The column role separates the two rows of a pair itself: pair pp = h
         // holds rows (h, l even) and (h, l odd), and with Z its spectrum,
         //   Xa[k] = (Z[k] + conj Z[N-k]) / 2,  Xb[k] = -i (Z[k] - conj Z[N-k]) / 2.
