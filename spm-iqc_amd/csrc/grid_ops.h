@@ -427,8 +427,8 @@ template <typename T, int N> struct SOpGradFlowStep {
     C2<T> *U;
     long ubs; // batched: member stride of U
     // U is read and written with non-temporal hints: set when U and W together
-    // overflow the last-level cache, so W (read again by the next three passes)
-    // keeps it; when both fit, the hints only cost
+    // overflow the last-level cache, so W (read again by the next pass) keeps
+    // it; when both fit, the hints only cost
     bool nt;
     __device__ __forceinline__ SOpGradFlowStep bind(int m) const
     {
