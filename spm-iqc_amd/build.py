@@ -65,6 +65,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
 
 COMPAT_LIB = os.path.join(LIBDIR, "libspm_iqc_compat.so")
 STAGE1 = os.path.join(LIBDIR, "spm_stage1")
+STAGE2 = os.path.join(LIBDIR, "spm_stage2")
 
 
 def _run(cmd):
@@ -74,12 +75,14 @@ def _run(cmd):
 
 
 def _build_compat():
-    """The reference-signature C++ layer over the C ABI, and the stage-1 driver."""
+    """The reference-signature C++ layer over the C ABI, and the stage-1 and stage-2 drivers."""
     src = os.path.join(HERE, "compat")
     host = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include")]
     _run(host + ["-shared", "-o", COMPAT_LIB, os.path.join(src, "compat.cpp"), "-L", LIBDIR, "-lspm_iqc",
                  "-Wl,-rpath,$ORIGIN"])
     _run(host + ["-o", STAGE1, os.path.join(src, "main_spm.cpp"), "-L", LIBDIR, "-lspm_iqc_compat", "-lspm_iqc",
+                 "-Wl,-rpath,$ORIGIN"])
+    _run(host + ["-o", STAGE2, os.path.join(src, "main_grad.cpp"), "-L", LIBDIR, "-lspm_iqc_compat", "-lspm_iqc",
                  "-Wl,-rpath,$ORIGIN"])
 
 

@@ -1,6 +1,6 @@
 """The reference-signature C++ layer (include/compat/) on the GPU: each
 function against the CPU oracle through a C++ test program, and the stage-1
-driver (main_SPM.cpp's call sequence) end to end."""
+and stage-2 drivers (the call sequences of main_SPM.cpp and grad.cpp) end to end."""
 import json
 import os
 import subprocess
@@ -53,3 +53,33 @@ def test_stage1_driver(seeds, tmp_path):
     assert np.isfinite(res["E_min"]) and np.isfinite(res["E_saddle"])
     saddle = np.loadtxt(out_file)
     assert saddle.shape == (N ** 3,) and np.isfinite(saddle).all()
+
+
+def test_stage2_driver(seeds, tmp_path):
+    """saddle file -> its modes -> the minimum it leads to -> the modes there (grad.cpp's
+    sequence). The first run computes the saddle's modes and saves them; the second finds the
+    file and reads them back (grad.cpp:92-113), and must reach the same minimum."""
+    N = 16
+    phi = seeds.gaussian(N, 0.05, 0)
+    src = tmp_path / "phi0.txt"
+    np.savetxt(src, phi.ravel(), fmt="%.16e ")
+    s1 = tmp_path / "S1.txt"
+    _run([os.path.join(ROOT, "spm-iqc_amd", "lib", "spm_stage1"), "--N", str(N), "--phi", str(src), "--k", "2",
+          "--lobpcg", "2", "--maxiter", "20", "--tol", "1e-9", "--out", str(s1)], cwd=str(tmp_path))
+    exe = os.path.join(ROOT, "spm-iqc_amd", "lib", "spm_stage2")
+    vsad = tmp_path / "V1_saddle.txt"
+    res = []
+    for tag in ("a", "b"):
+        out = _run([exe, "--N", str(N), "--in", str(s1), "--vsaddle", str(vsad), "--k", "3", "--lobpcg1", "2",
+                    "--lobpcg2", "2", "--maxiter", "200", "--min", str(tmp_path / ("Min_%s.txt" % tag)),
+                    "--vmin", str(tmp_path / ("V0_Min_%s.txt" % tag))], cwd=str(tmp_path))
+        res.append(json.loads([l for l in out.splitlines() if l.startswith("{")][-1]))
+    assert res[0]["modes_loaded"] == 0 and res[1]["modes_loaded"] == 1
+    assert np.loadtxt(vsad).shape == (3 * N ** 3,)
+    mins = [np.loadtxt(tmp_path / ("Min_%s.txt" % t)) for t in ("a", "b")]
+    assert mins[0].shape == (N ** 3,) and np.isfinite(mins[0]).all()
+    # "%.16e" round-trips a double, so the read-back modes are the computed ones
+    assert np.array_equal(mins[0], mins[1]) and res[0]["E_min"] == res[1]["E_min"]
+    assert res[0]["E_min"] < res[0]["E_saddle"]  # gradient flow from the saddle goes downhill
+    vmin = np.loadtxt(tmp_path / "V0_Min_a.txt")
+    assert vmin.shape == (3 * N ** 3,) and np.isfinite(vmin).all()
