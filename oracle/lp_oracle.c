@@ -742,6 +742,213 @@ int lpo_lobpcg_inip(lpo_ctx *c, const lpo_cplx *x, lpo_cplx *Vio, int k, int max
     return iter;
 }
 
+/* -------------------------------------------------------------- HiSD */
+
+/* hiosd_lobpcg_sieh (HISD.cpp:86-485): the high-index saddle dynamics of an
+ * index-k saddle with a LOBPCG sweep per step for the k unstable directions V.
+ * x, Vio (k fields) in place; returns the iteration count, *res_out the last
+ * nrmp(ngrad_cam(x)).  EigenSolver's complex-eigenvalue exit (:363-372) cannot
+ * occur here: the symmetrised Rayleigh-Ritz matrix goes to the Jacobi solver. */
+int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double dt, double betat, double betau,
+                          double epsf, int maxiter, double *res_out)
+{
+    const long R = c->R;
+    const double Tau = c->Tau, Gamma = c->Gamma;
+    lpo_cplx **V = (lpo_cplx **)malloc(sizeof(void *) * 6 * k);
+    lpo_cplx **U = V + k, **W = V + 2 * k, **Y = V + 3 * k, **Vp = V + 4 * k, **Up = V + 5 * k;
+    for (int i = 0; i < 6 * k; ++i) V[i] = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+    for (int i = 0; i < k; ++i)
+        for (long j = 0; j < R; ++j) {
+            V[i][j][0] = Vio[(long)i * R + j][0];
+            V[i][j][1] = 0.0;
+        }
+    double *alpha = (double *)calloc(k, sizeof(double));
+    int *emp = (int *)malloc(sizeof(int) * 3 * k);
+    double *Pn = (double *)malloc(sizeof(double) * 9 * k * k);
+    double *Pw = (double *)malloc(sizeof(double) * 9 * k * k);
+    double *Pz = (double *)malloc(sizeof(double) * 9 * k * k);
+    double *val = (double *)malloc(sizeof(double) * 3 * k);
+    lpo_cplx **UU = (lpo_cplx **)malloc(sizeof(void *) * 3 * k);
+    lpo_cplx **YY = (lpo_cplx **)malloc(sizeof(void *) * 3 * k);
+    double *Vm = (double *)malloc(sizeof(double) * (size_t)k * R);
+    lpo_cplx *rho = (lpo_cplx *)calloc(R, sizeof(lpo_cplx)), *f = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+    lpo_cplx *g = (lpo_cplx *)calloc(R, sizeof(lpo_cplx)), *gp = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+    lpo_cplx *Dx = (lpo_cplx *)calloc(R, sizeof(lpo_cplx)), *Dg = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+    lpo_cplx *tmpp = (lpo_cplx *)calloc(R, sizeof(lpo_cplx)), *tmppF = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+    lpo_cplx *nlnR = (lpo_cplx *)calloc(R, sizeof(lpo_cplx)), *nlnF = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+    lpo_cplx *xp = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
+
+    for (int i = 0; i < k; ++i) { /* :151-156 */
+        orthoi_Vi(c, V, i);
+        const double temp1 = lpo_nrmp(c, V[i]);
+        scale_c(c, V[i], 1.0 / temp1);
+    }
+    /* PV_Cplx(f, V, tmpp, k) (BasicOperators.cpp:360-371): tmpp = sum_i <f, V_i> V_i */
+#define PV_F()                                                                                                   \
+    do {                                                                                                         \
+        memset(tmpp, 0, sizeof(lpo_cplx) * R);                                                                   \
+        for (int i_ = 0; i_ < k; ++i_) {                                                                         \
+            const double t_ = lpo_inpfp(c, f, V[i_]);                                                            \
+            for (long j_ = 0; j_ < R; ++j_) {                                                                    \
+                tmpp[j_][0] = tmpp[j_][0] + t_ * V[i_][j_][0];                                                   \
+                tmpp[j_][1] = 0.0;                                                                               \
+            }                                                                                                    \
+        }                                                                                                        \
+    } while (0)
+    lpo_ngrad_cam(c, x, f); /* :182-186 */
+    PV_F();
+    for (long j = 0; j < R; ++j) { /* Dx = f - 2 tmpp, then FuncRealAddAConst(Dx, dt) */
+        Dx[j][0] = 1.0 * f[j][0] + -2.0 * tmpp[j][0];
+        Dx[j][1] = 1.0 * f[j][1] + -2.0 * tmpp[j][1];
+        Dx[j][0] += dt;
+    }
+    double res = 0.0;
+    int iter = 0;
+    while (iter < maxiter) {
+        for (int i = 0; i < 3 * k; ++i) emp[i] = 1;
+        PV_F(); /* :193-195 */
+        scale_c(c, tmpp, 2.0);
+        for (long j = 0; j < R; ++j) { /* g = f - tmpp; Dg = g - gp; gp = g (:199-204) */
+            g[j][0] = 1.0 * f[j][0] + -1.0 * tmpp[j][0];
+            g[j][1] = 1.0 * f[j][1] + -1.0 * tmpp[j][1];
+            Dg[j][0] = 1.0 * g[j][0] + -1.0 * gp[j][0];
+            Dg[j][1] = 1.0 * g[j][1] + -1.0 * gp[j][1];
+            gp[j][0] = 1.0 * g[j][0];
+            gp[j][1] = 1.0 * g[j][1];
+        }
+        double bta = fabs(lpo_inpfp(c, Dx, Dg) / lpo_inpfp(c, Dg, Dg)); /* :205-212 */
+        bta = bta < betat * dt ? bta : betat * dt;
+        bta = bta > betau * dt ? bta : betau * dt;
+        for (long j = 0; j < R; ++j) {
+            xp[j][0] = 1.0 * x[j][0];
+            xp[j][1] = 1.0 * x[j][1];
+        }
+        /* the semi-implicit step with the reflected force, real part of the spectrum (:215-268) */
+#pragma omp parallel for num_threads(c->nthreads) schedule(static)
+        for (long i = 0; i < R; ++i) {
+            c->u2[i][0] = x[i][0] * x[i][0];
+            c->u2[i][1] = 0.0;
+            c->u3[i][0] = c->u2[i][0] * x[i][0];
+            c->u3[i][1] = 0.0;
+        }
+        lpo_my_fft(c, x, rho);
+#pragma omp parallel for num_threads(c->nthreads) schedule(static)
+        for (long i = 0; i < R; ++i) {
+            nlnR[i][0] = Tau * x[i][0] + Gamma * c->u2[i][0] - c->u3[i][0];
+            nlnR[i][1] = 0.0;
+        }
+        const double mean_all = lpo_mean(c, nlnR);
+        for (long i = 0; i < R; ++i) nlnR[i][0] += -mean_all;
+        lpo_my_fft(c, nlnR, nlnF);
+        lpo_my_fft(c, tmpp, tmppF);
+#pragma omp parallel for num_threads(c->nthreads) schedule(static)
+        for (long i = 0; i < R; ++i) {
+            rho[i][0] = (rho[i][0] + bta * nlnF[i][0] - bta * tmppF[i][0]) / (1 + bta * c->G[i]);
+            rho[i][1] = 0.0;
+        }
+        rho[0][0] = 0.0;
+        rho[0][1] = 0.0;
+        lpo_my_ifft(c, rho, x);
+        for (long j = 0; j < R; ++j) { /* :274-278 */
+            Dx[j][0] = x[j][0] - xp[j][0];
+            Dx[j][1] = 0.0;
+        }
+        /* the LOBPCG sweep at the new x (:282-441), as in hiosd_lobpcg_inip */
+        for (int i = 0; i < k; ++i) {
+            lpo_hv_cam(c, x, V[i], U[i]);
+            alpha[i] = lpo_inpfp(c, V[i], U[i]);
+            for (long j = 0; j < R; ++j) {
+                W[i][j][0] = 1 * U[i][j][0] + -alpha[i] * V[i][j][0];
+                W[i][j][1] = 1 * U[i][j][1] + -alpha[i] * V[i][j][1];
+            }
+            lpo_precond_camnew(c, W[i]);
+            orthoi_v2V(c, W[i], V, k);
+            orthoi_Vi(c, W, i);
+            const double nrmW = lpo_nrmp(c, W[i]);
+            if (nrmW > 1e-8) {
+                scale_c(c, W[i], 1.0 / nrmW);
+                lpo_hv_cam(c, x, W[i], Y[i]);
+            } else {
+                memset(W[i], 0, sizeof(lpo_cplx) * R);
+                memset(Y[i], 0, sizeof(lpo_cplx) * R);
+                emp[k + i] = 0;
+            }
+        }
+        for (int i = 0; i < k; ++i) {
+            orthoi_v2V(c, Vp[i], V, k);
+            orthoi_v2V(c, Vp[i], W, k);
+            orthoi_Vi(c, Vp, i);
+            const double nrmVpi = lpo_nrmp(c, Vp[i]);
+            if (nrmVpi > 1e-8) {
+                scale_c(c, Vp[i], 1.0 / nrmVpi);
+                lpo_hv_cam(c, x, Vp[i], Up[i]);
+            } else {
+                memset(Vp[i], 0, sizeof(lpo_cplx) * R);
+                memset(Up[i], 0, sizeof(lpo_cplx) * R);
+                emp[2 * k + i] = 0;
+            }
+        }
+        int m = 0;
+        for (int i = 0; i < 3 * k; ++i)
+            if (emp[i] == 1) {
+                UU[m] = i < k ? V[i] : (i < 2 * k ? W[i - k] : Vp[i - 2 * k]);
+                YY[m] = i < k ? U[i] : (i < 2 * k ? Y[i - k] : Up[i - 2 * k]);
+                ++m;
+            }
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) {
+                double s;
+                const lpo_cplx *ua = UU[a], *yb = YY[b];
+                BLOCK_SUM(c, ua[i][0] * yb[i][0], s);
+                Pn[a * m + b] = s;
+            }
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) Pw[a * m + b] = (Pn[b * m + a] + Pn[a * m + b]) * 0.5 / (int)R;
+        lpo_sym_eig(m, Pw, val, Pz);
+        for (long j = 0; j < R; ++j)
+            for (int i = 0; i < k; ++i) {
+                double s = 0.0;
+                for (int a = 0; a < m; ++a) s += Pz[a * m + i] * UU[a][j][0];
+                Vm[(long)i * R + j] = s;
+            }
+        for (int i = 0; i < k; ++i)
+            for (long j = 0; j < R; ++j) {
+                Vp[i][j][0] = V[i][j][0]; Vp[i][j][1] = 0.0;
+                Up[i][j][0] = U[i][j][0]; Up[i][j][1] = 0.0;
+            }
+        for (int i = 0; i < k; ++i)
+            for (long j = 0; j < R; ++j) {
+                V[i][j][0] = Vm[(long)i * R + j];
+                V[i][j][1] = 0.0;
+            }
+        for (int i = 0; i < k; ++i) {
+            const double temp1 = lpo_mean(c, V[i]);
+            for (long j = 0; j < R; ++j) { V[i][j][0] = V[i][j][0] - temp1; V[i][j][1] = 0.0; }
+        }
+        for (int i = 0; i < k; ++i) {
+            orthoi_v2V(c, V[i], V, i);
+            const double temp1 = lpo_nrmp(c, V[i]);
+            scale_c(c, V[i], 1.0 / temp1);
+        }
+        lpo_ngrad_cam(c, x, f); /* :443-447 */
+        res = lpo_nrmp(c, f);
+        iter = iter + 1;
+        if (res < epsf) break;
+    }
+#undef PV_F
+    for (int i = 0; i < k; ++i)
+        for (long j = 0; j < R; ++j) {
+            Vio[(long)i * R + j][0] = V[i][j][0];
+            Vio[(long)i * R + j][1] = 0.0;
+        }
+    if (res_out) *res_out = res;
+    for (int i = 0; i < 6 * k; ++i) free(V[i]);
+    free(V); free(alpha); free(emp); free(Pn); free(Pw); free(Pz); free(val); free(UU); free(YY); free(Vm);
+    free(rho); free(f); free(g); free(gp); free(Dx); free(Dg); free(tmpp); free(tmppF); free(nlnR); free(nlnF);
+    free(xp);
+    return iter;
+}
+
 /* -------------------------------------------------------------- SPM */
 
 /* SPM.cpp:15-32 */

@@ -81,6 +81,11 @@ void lpo_MinMax_Forces(lpo_ctx *c, lpo_cplx *x, const lpo_cplx *v);
 int lpo_Spring_Pairing(lpo_ctx *c, lpo_cplx *Phi0, lpo_cplx *Phi1, int maxiter,
                        double tol, double *res_out);
 
+/* HISD.cpp:86-485 (hiosd_lobpcg_sieh): x and V (k fields) in place; returns
+ * the number of iterations; *res_out receives the last nrmp(ngrad_cam(x)). */
+int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *V, int k, double dt, double betat, double betau,
+                          double epsf, int maxiter, double *res_out);
+
 /* Symmetric eigensolver for the small Rayleigh-Ritz problem (cyclic Jacobi).
  * A is n*n row-major, overwritten; w receives eigenvalues ascending, Z the
  * matching eigenvectors as columns (row-major n*n). */

@@ -60,6 +60,7 @@ def lib():
             "lpo_Natural_Forces": (None, [_c, _c, _c]),
             "lpo_MinMax_Forces": (None, [_c, _c, _c]),
             "lpo_Spring_Pairing": (_i, [_c, _c, _c, _i, _d, _c]),
+            "lpo_hiosd_lobpcg_sieh": (_i, [_c, _c, _c, _i, _d, _d, _d, _d, _i, _c]),
             "lpo_sym_eig": (None, [_i, _c, _c, _c]),
         }
         for name, (res, args) in sig.items():
@@ -224,4 +225,14 @@ class Oracle:
         """In place on Phi0, Phi1; returns (iters, last residual)."""
         r = np.zeros(1)
         it = lib().lpo_Spring_Pairing(self.h, _p(Phi0), _p(Phi1), int(maxiter), float(tol), _p(r))
+        return it, float(r[0])
+
+    # --- HiSD --------------------------------------------------------------
+    def hiosd_lobpcg_sieh(self, x, V, dt=0.8, betat=8.0, betau=0.02, epsf=1e-7, maxiter=10000):
+        """HISD.cpp:86-485, in place on x and V (k, N, N, N); returns (iters, last residual)."""
+        assert x.flags["C_CONTIGUOUS"] and x.dtype == np.complex128
+        assert V.flags["C_CONTIGUOUS"] and V.dtype == np.complex128
+        r = np.zeros(1)
+        it = lib().lpo_hiosd_lobpcg_sieh(self.h, _p(x), _p(V), V.shape[0], float(dt), float(betat), float(betau),
+                                         float(epsf), int(maxiter), _p(r))
         return it, float(r[0])

@@ -270,3 +270,58 @@ def test_spring_pairing_exit_semantics(po, seeds):
     a, b = p0.copy(), p1.copy()
     it, r = o.Spring_Pairing(a, b, 400, 1e-30)
     assert it < 400 and r == 0.0 and np.abs(b.real).max() > 1e100
+
+
+# ---------------------------------------------------------------- HiSD
+def _flip(a):
+    """a(-r) on the periodic grid."""
+    return np.roll(a[::-1, ::-1, ::-1], 1, axis=(0, 1, 2))
+
+
+def test_hisd_step_is_gradient_flow_for_even_field_and_odd_mode(po, seeds):
+    """hiosd_lobpcg_sieh (HISD.cpp:188-268) on an even field with an odd mode v:
+    <f, v> = 0, so the reflected force is f itself, and the first Barzilai-Borwein
+    step is |<f + dt, f>| / <f, f> = 1 (mean(f) = 0). The first x-step is then one
+    get_min_u_F step with dt = 1 (LPsrc.cpp:56-84)."""
+    N = 16
+    y = seeds.gaussian(N, 0.05, 3)
+    x0 = 0.5 * (y + _flip(y))
+    z = seeds.gaussian(N, 1.0, 4)
+    v = 0.5 * (z - _flip(z))
+    v /= np.sqrt(np.mean(v * v))
+    o = po.Oracle(N, dt=1.0)
+    x = (x0 + 0j).copy()
+    V = (v + 0j).reshape((1, N, N, N)).copy()
+    it, res = o.hiosd_lobpcg_sieh(x, V, dt=0.8, maxiter=1, epsf=-1.0)
+    assert it == 1
+    uR = (x0 + 0j).copy()
+    uF = o.my_fft(uR)
+    o.get_min_u_F(uR, uF, 1, -1.0)
+    assert np.abs(x.real - uR.real).max() < 1e-14
+    assert np.abs(x.imag).max() < 1e-18  # rounding of the complex inverse transform
+
+
+def test_hisd_step_keeps_the_even_part(po, seeds):
+    """The x-step keeps only the real part of the new spectrum (HISD.cpp:254-258),
+    so from any start the field after one step is even: x(-r) = x(r)."""
+    N = 12
+    o = po.Oracle(N)
+    x = (seeds.gaussian(N, 0.05, 5) + 0j).copy()
+    V = (seeds.gaussian(N, 1.0, 6) + 0j).reshape((1, N, N, N)).copy()
+    o.hiosd_lobpcg_sieh(x, V, maxiter=1, epsf=-1.0)
+    assert np.abs(x.real - _flip(x.real)).max() < 1e-15
+
+
+def test_hisd_eigen_sweep_is_lobpcg(po, seeds):
+    """The eigen-direction part of a HiSD step (HISD.cpp:282-441) is one
+    hiosd_lobpcg_inip sweep (LOBPCG.cpp:262-449) at the new x, started from the
+    same block with Vp = 0."""
+    N, k = 12, 2
+    o = po.Oracle(N)
+    x = (seeds.gaussian(N, 0.05, 7) + 0j).copy()
+    V0 = np.stack([seeds.gaussian(N, 1.0, 8 + i) for i in range(k)]) + 0j
+    V = V0.copy()
+    o.hiosd_lobpcg_sieh(x, V, maxiter=1, epsf=-1.0)
+    W = V0.copy()
+    o.lobpcg_inip(x, W, maxiter=1)
+    assert np.abs(V - W).max() < 1e-12
