@@ -147,6 +147,19 @@ int spm_gram(spm_ctx *c, const void *X, int ma, const void *Y, int mb, double *o
 /* rayleighq (BasicOperators.cpp:174-185): <v, H v> / <v, v>. */
 int spm_rayleighq(spm_ctx *c, const void *phi, const void *v, double *out);
 
+/* hiosd_lobpcg_sieh (HISD.cpp:86-485): high-index saddle dynamics for an
+ * index-k saddle, in place on x and on its k unstable directions V (k
+ * contiguous fields).  Each step is the semi-implicit step of x under the force
+ * with its components along V reflected, with a Barzilai-Borwein step size
+ * clamped to [betau dt, betat dt] and only the real part of the new spectrum
+ * kept (as the reference does), then one LOBPCG sweep for V at the new x.  Stops
+ * when nrmp(ngrad(x)) < epsf or after maxiter steps; *iters and *res (may be
+ * NULL) receive the step count and the last residual.  The reference's driver
+ * values are dt 0.8, betat 8, betau 0.02, epsf 1e-7, maxiter 10000
+ * (HISD.cpp:16-22).  1 <= k <= 8. */
+int spm_hisd(spm_ctx *c, void *x, void *V, int k, double dt, double betat, double betau, double epsf, int maxiter,
+             int *iters, double *res);
+
 /* --- SPM */
 /* get_Tv (SPM.cpp:15-32): v = (phi1 - phi0) / nrmp(phi1 - phi0). */
 int spm_get_tv(spm_ctx *c, const void *phi0, const void *phi1, void *v);

@@ -73,6 +73,7 @@ SIGNATURES = {
     "spm_hv": (_i, [_c, _c, _c, _c]),
     "spm_lobpcg_init": (_i, [_c, _c, _c, _i]),
     "spm_lobpcg": (_i, [_c, _c, _c, _i, _i, _c]),
+    "spm_hisd": (_i, [_c, _c, _c, _i, _d, _d, _d, _d, _i, _c, _c]),
     "spm_rayleighq": (_i, [_c, _c, _c, _pd]),
     "spm_gram": (_i, [_c, _c, _i, _c, _i, _c]),
     "spm_get_tv": (_i, [_c, _c, _c, _c]),
@@ -347,6 +348,17 @@ class Grid:
         _check(lib().spm_lobpcg(self.h, self._real(phi), self._block(V), k, int(maxiter), ritz.ctypes.data_as(_c)),
                "spm_lobpcg")
         return ritz
+
+    def hisd(self, x, V, dt=0.8, betat=8.0, betau=0.02, epsf=1e-7, maxiter=10000):
+        """hiosd_lobpcg_sieh (HISD.cpp:86-485) on x and V (k, N, N, N) in place; returns
+        (steps, last residual)."""
+        self._stream()
+        k = int(V.shape[0])
+        it = np.zeros(1, dtype=np.int32)
+        res = np.zeros(1)
+        _check(lib().spm_hisd(self.h, self._real(x), self._block(V), k, float(dt), float(betat), float(betau),
+                              float(epsf), int(maxiter), it.ctypes.data_as(_c), res.ctypes.data_as(_c)), "spm_hisd")
+        return int(it[0]), float(res[0])
 
     def gram(self, X, Y):
         """inpfp of every pair of fields of the blocks X (ma, N, N, N), Y (mb, N, N, N)."""

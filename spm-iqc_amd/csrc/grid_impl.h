@@ -251,6 +251,23 @@ struct GridImpl final : GridIface {
         return c.nb_rows;
     }
 
+    void hisd_step(Ctx &c, void *x, const void *tmpp, double bta) override
+    {
+        const OpHisdIn<T> in{c.Tau, c.Gamma, bta, (const T *)tmpp};
+        const SOpHisdStep<T, N> st{1.0 / (double)c.R, bta, c.Gtab};
+        if constexpr (N == 64 || N == 128 || N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(c, x, in, nullptr, [] {}, st, OpStore<T>{(T *)x}, nullptr);
+                return;
+            }
+        }
+        rows<false, true>(c, x, c.W, in, nullptr);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, st, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)x}, nullptr);
+    }
+
     void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef, ForceSpec *fs) override
     {
         force_any(c, x, v, s, coef, fs, true);

@@ -309,6 +309,23 @@ template <typename T> struct OpForceIn {
     __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, v[g], g, acc); }
 };
 
+// HiSD step input (HISD.cpp:215-250): y = x + bta (Tau x + Gamma x^2 - x^3) -
+// bta tmpp, tmpp the reflected part of the force (its mean only moves the DC,
+// which the step zeroes).
+template <typename T> struct OpHisdIn {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    double Tau, Gamma, bta;
+    const T *tmpp;
+    __device__ __forceinline__ const T *aux() const { return tmpp; }
+    __device__ __forceinline__ T operator()(T x, T tg, long, double *) const
+    {
+        const double u = (double)x, u2 = u * u, u3 = u2 * u;
+        return (T)(u + bta * (Tau * u + Gamma * u2 - u3) - bta * (double)tg);
+    }
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, tmpp[g], g, acc); }
+};
+
 // ======================================================================
 // Spectral operators applied inside the i-axis column pass.
 // op(v, s, i, j, kz, acc): v = (unnormalised forward) value at spectrum index
@@ -351,6 +368,25 @@ template <typename T, int N> struct SOpMulTab {
         const double f = n2 == 0 ? 0.0 : sc * tab[n2];
         v.x = (T)((double)v.x * f);
         v.y = (T)((double)v.y * f);
+    }
+};
+
+// HiSD step (HISD.cpp:252-262): v <- sc Re(v) / (1 + bta G), DC = 0.  Only the
+// real part of each coefficient is kept, as the reference does, so the new
+// field is the even part of the transform's result.
+template <typename T, int N> struct SOpHisdStep {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    static constexpr bool PRELOAD = false, FINALIZE = false;
+    double sc, bta;
+    const double *G;
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T>, long, int i, int j, int kz, double *) const
+    {
+        const int a = kfold(i, N), b = kfold(j, N);
+        const int n2 = a * a + b * b + kz * kz;
+        const double f = n2 == 0 ? 0.0 : sc / (1.0 + bta * G[n2]);
+        v.x = (T)((double)v.x * f);
+        v.y = (T)0;
     }
 };
 

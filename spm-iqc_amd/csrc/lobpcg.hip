@@ -114,115 +114,226 @@ int lobpcg_init(spm_ctx *h, const void *start, void *V, int k)
     return block_ortho(h, Vp.data(), k, nullptr, 0, -1.0, (bool *)keep.data());
 }
 
+// The block state of a LOBPCG iteration: V (the caller's fields or, after the
+// first Ritz update, workspace fields), U = H V, W, Y = H W, Vp, Up = H Vp, and a
+// spare block T; keep flags of V, W, Vp.
+struct LobBlock {
+    std::vector<void *> V, U, W, Y, Vp, Up, T;
+    std::vector<char> keepbuf;
+    bool *keepV() { return (bool *)keepbuf.data(); }
+    bool *keepW() { return (bool *)keepbuf.data() + V.size(); }
+    bool *keepP() { return (bool *)keepbuf.data() + 2 * V.size(); }
+};
+
+static std::vector<const void *> cv(const std::vector<void *> &a) { return std::vector<const void *>(a.begin(), a.end()); }
+
+// V = the k fields at Vout, U..T the first 6k fields of the workspace (zeroed)
+static int lob_block(spm_ctx *h, void *Vout, int k, size_t extra_fields, LobBlock &b)
+{
+    Ctx &c = h->c;
+    const size_t fb = (size_t)c.R * c.esz;
+    RCHK(ensure_ws(h, (6 * (size_t)k + extra_fields) * fb));
+    b.V.resize(k);
+    b.U.resize(k);
+    b.W.resize(k);
+    b.Y.resize(k);
+    b.Vp.resize(k);
+    b.Up.resize(k);
+    b.T.resize(k);
+    for (int i = 0; i < k; ++i) {
+        b.V[i] = fld(Vout, i, c);
+        b.U[i] = fld(h->ws, 0 * k + i, c);
+        b.W[i] = fld(h->ws, 1 * k + i, c);
+        b.Y[i] = fld(h->ws, 2 * k + i, c);
+        b.Vp[i] = fld(h->ws, 3 * k + i, c);
+        b.Up[i] = fld(h->ws, 4 * k + i, c);
+        b.T[i] = fld(h->ws, 5 * k + i, c);
+    }
+    b.keepbuf.assign(3 * k, 0);
+    HIPCHK(hipMemsetAsync(h->ws, 0, (6 * (size_t)k + extra_fields) * fb, c.stream));
+    return SPM_OK;
+}
+
+// One iteration of hiosd_lobpcg_inip (LOBPCG.cpp:279-449) at phi; ritz (k) may be null.
+static int lobpcg_sweep(spm_ctx *h, const void *phi, LobBlock &b, double *ritz)
+{
+    Ctx &c = h->c;
+    const int k = (int)b.V.size();
+    std::vector<void *> &V = b.V, &U = b.U, &W = b.W, &Y = b.Y, &Vp = b.Vp, &Up = b.Up, &T = b.T;
+    bool *keepV = b.keepV(), *keepW = b.keepW(), *keepP = b.keepP();
+    for (int i = 0; i < k; ++i) c.g->hv(c, phi, V[i], U[i]);
+    RCHK(spm_launch_status("lobpcg hv"));
+    std::vector<double> VU(k * k);
+    RCHK(block_gram(h, cv(V).data(), k, cv(U).data(), k, VU.data()));
+    {   // W = U - alpha V  (FuncsLinear2Cplx, :284)
+        std::vector<const void *> in(2 * k);
+        std::vector<double> coef(2 * k * k, 0.0);
+        for (int i = 0; i < k; ++i) {
+            in[i] = U[i];
+            in[k + i] = V[i];
+            coef[i * k + i] = 1.0;
+            coef[(k + i) * k + i] = -VU[i * k + i];
+        }
+        RCHK(block_combine(h, in.data(), 2 * k, W.data(), k, coef.data(), nullptr));
+    }
+    for (int i = 0; i < k; ++i) c.g->precond(c, W[i]);
+    RCHK(spm_launch_status("lobpcg precond"));
+    RCHK(block_ortho(h, W.data(), k, cv(V).data(), k, 1e-8, keepW));
+    for (int i = 0; i < k; ++i) {
+        if (keepW[i]) c.g->hv(c, phi, W[i], Y[i]);
+        else RCHK(field_zero(h, Y[i]));
+    }
+    {
+        std::vector<const void *> B(2 * k);
+        for (int i = 0; i < k; ++i) {
+            B[i] = V[i];
+            B[k + i] = W[i];
+        }
+        RCHK(block_ortho(h, Vp.data(), k, B.data(), 2 * k, 1e-8, keepP));
+    }
+    for (int i = 0; i < k; ++i) {
+        if (keepP[i]) c.g->hv(c, phi, Vp[i], Up[i]);
+        else RCHK(field_zero(h, Up[i]));
+    }
+    RCHK(spm_launch_status("lobpcg hv"));
+    // Rayleigh-Ritz on the kept basis (get_Merge / get_mat, :332-362)
+    std::vector<const void *> UU, YY;
+    for (int i = 0; i < k; ++i) {
+        UU.push_back(V[i]);
+        YY.push_back(U[i]);
+    }
+    for (int i = 0; i < k; ++i)
+        if (keepW[i]) {
+            UU.push_back(W[i]);
+            YY.push_back(Y[i]);
+        }
+    for (int i = 0; i < k; ++i)
+        if (keepP[i]) {
+            UU.push_back(Vp[i]);
+            YY.push_back(Up[i]);
+        }
+    const int m = (int)UU.size();
+    std::vector<double> Pn(m * m), Ps(m * m), Z, vals;
+    RCHK(block_gram(h, UU.data(), m, YY.data(), m, Pn.data()));
+    for (int a = 0; a < m; ++a)
+        for (int bb = 0; bb < m; ++bb) Ps[a * m + bb] = (Pn[bb * m + a] + Pn[a * m + bb]) * 0.5;
+    sym_eig(m, Ps, vals, Z);
+    // T = eta1^T UU (:402-403)
+    std::vector<double> coef(m * k);
+    for (int a = 0; a < m; ++a)
+        for (int i = 0; i < k; ++i) coef[a * k + i] = Z[a * m + i];
+    RCHK(block_combine(h, UU.data(), m, T.data(), k, coef.data(), nullptr));
+    // Vp <- V, Up <- U (:391-400); V <- T
+    std::vector<void *> oldVp = Vp, oldUp = Up;
+    Vp = V;
+    Up = U;
+    V = T;
+    T = oldVp;
+    U = oldUp;
+    // mean removal (:429-441), then orthonormalise in order (:443-449)
+    std::vector<double> mean(k), eye(k * k, 0.0), shift(k);
+    RCHK(block_means(h, cv(V).data(), k, mean.data()));
+    for (int i = 0; i < k; ++i) {
+        eye[i * k + i] = 1.0;
+        shift[i] = -mean[i];
+    }
+    RCHK(block_combine(h, cv(V).data(), k, V.data(), k, eye.data(), shift.data()));
+    RCHK(block_ortho(h, V.data(), k, nullptr, 0, -1.0, keepV));
+    if (ritz)
+        for (int i = 0; i < k; ++i) ritz[i] = vals[i];
+    return SPM_OK;
+}
+
+// V back to the caller's fields at Vout when the last Ritz update left it in the workspace
+static int lob_copy_out(spm_ctx *h, LobBlock &b, void *Vout)
+{
+    Ctx &c = h->c;
+    for (int i = 0; i < (int)b.V.size(); ++i)
+        if (b.V[i] != fld(Vout, i, c)) RCHK(field_copy(h, fld(Vout, i, c), b.V[i]));
+    return SPM_OK;
+}
+
 // hiosd_lobpcg_inip (LOBPCG.cpp:207-487)
 int lobpcg_run(spm_ctx *h, const void *phi, void *Vout, int k, int maxiter, double *ritz)
 {
     Ctx &c = h->c;
-    const size_t fb = (size_t)c.R * c.esz;
-    RCHK(ensure_ws(h, 6 * (size_t)k * fb));
-    // buffers: V (caller), U, W, Y, Vp, Up, T in the workspace
-    std::vector<void *> V(k), U(k), W(k), Y(k), Vp(k), Up(k), T(k);
-    for (int i = 0; i < k; ++i) {
-        V[i] = fld(Vout, i, c);
-        U[i] = fld(h->ws, 0 * k + i, c);
-        W[i] = fld(h->ws, 1 * k + i, c);
-        Y[i] = fld(h->ws, 2 * k + i, c);
-        Vp[i] = fld(h->ws, 3 * k + i, c);
-        Up[i] = fld(h->ws, 4 * k + i, c);
-        T[i] = fld(h->ws, 5 * k + i, c);
-    }
-    HIPCHK(hipMemsetAsync(h->ws, 0, 6 * (size_t)k * fb, c.stream));
-    auto cv = [](const std::vector<void *> &a) { return std::vector<const void *>(a.begin(), a.end()); };
-    std::vector<char> keepbuf(3 * k);
-    bool *keepV = (bool *)keepbuf.data(), *keepW = keepV + k, *keepP = keepW + k;
-    RCHK(block_ortho(h, V.data(), k, nullptr, 0, -1.0, keepV)); // :262-267
-    std::vector<double> vals;
-    for (int iter = 0; iter < maxiter; ++iter) {
-        for (int i = 0; i < k; ++i) c.g->hv(c, phi, V[i], U[i]);
-        RCHK(spm_launch_status("lobpcg hv"));
-        std::vector<double> VU(k * k);
-        RCHK(block_gram(h, cv(V).data(), k, cv(U).data(), k, VU.data()));
-        {   // W = U - alpha V  (FuncsLinear2Cplx, :284)
-            std::vector<const void *> in(2 * k);
-            std::vector<double> coef(2 * k * k, 0.0);
-            for (int i = 0; i < k; ++i) {
-                in[i] = U[i];
-                in[k + i] = V[i];
-                coef[i * k + i] = 1.0;
-                coef[(k + i) * k + i] = -VU[i * k + i];
-            }
-            RCHK(block_combine(h, in.data(), 2 * k, W.data(), k, coef.data(), nullptr));
-        }
-        for (int i = 0; i < k; ++i) c.g->precond(c, W[i]);
-        RCHK(spm_launch_status("lobpcg precond"));
-        RCHK(block_ortho(h, W.data(), k, cv(V).data(), k, 1e-8, keepW));
-        for (int i = 0; i < k; ++i) {
-            if (keepW[i]) c.g->hv(c, phi, W[i], Y[i]);
-            else RCHK(field_zero(h, Y[i]));
-        }
-        {
-            std::vector<const void *> B(2 * k);
-            for (int i = 0; i < k; ++i) {
-                B[i] = V[i];
-                B[k + i] = W[i];
-            }
-            RCHK(block_ortho(h, Vp.data(), k, B.data(), 2 * k, 1e-8, keepP));
-        }
-        for (int i = 0; i < k; ++i) {
-            if (keepP[i]) c.g->hv(c, phi, Vp[i], Up[i]);
-            else RCHK(field_zero(h, Up[i]));
-        }
-        RCHK(spm_launch_status("lobpcg hv"));
-        // Rayleigh-Ritz on the kept basis (get_Merge / get_mat, :332-362)
-        std::vector<const void *> UU, YY;
-        for (int i = 0; i < k; ++i) {
-            UU.push_back(V[i]);
-            YY.push_back(U[i]);
-        }
-        for (int i = 0; i < k; ++i)
-            if (keepW[i]) {
-                UU.push_back(W[i]);
-                YY.push_back(Y[i]);
-            }
-        for (int i = 0; i < k; ++i)
-            if (keepP[i]) {
-                UU.push_back(Vp[i]);
-                YY.push_back(Up[i]);
-            }
-        const int m = (int)UU.size();
-        std::vector<double> Pn(m * m), Ps(m * m), Z;
-        RCHK(block_gram(h, UU.data(), m, YY.data(), m, Pn.data()));
-        for (int a = 0; a < m; ++a)
-            for (int b = 0; b < m; ++b) Ps[a * m + b] = (Pn[b * m + a] + Pn[a * m + b]) * 0.5;
-        sym_eig(m, Ps, vals, Z);
-        // T = eta1^T UU (:402-403)
-        std::vector<double> coef(m * k);
-        for (int a = 0; a < m; ++a)
-            for (int i = 0; i < k; ++i) coef[a * k + i] = Z[a * m + i];
-        RCHK(block_combine(h, UU.data(), m, T.data(), k, coef.data(), nullptr));
-        // Vp <- V, Up <- U (:391-400); V <- T
-        std::vector<void *> oldVp = Vp, oldUp = Up;
-        Vp = V;
-        Up = U;
-        V = T;
-        T = oldVp;
-        U = oldUp;
-        // mean removal (:429-441), then orthonormalise in order (:443-449)
-        std::vector<double> mean(k), eye(k * k, 0.0), shift(k);
-        RCHK(block_means(h, cv(V).data(), k, mean.data()));
-        for (int i = 0; i < k; ++i) {
-            eye[i * k + i] = 1.0;
-            shift[i] = -mean[i];
-        }
-        RCHK(block_combine(h, cv(V).data(), k, V.data(), k, eye.data(), shift.data()));
-        RCHK(block_ortho(h, V.data(), k, nullptr, 0, -1.0, keepV));
-        if (ritz)
-            for (int i = 0; i < k; ++i) ritz[i] = vals[i];
-    }
-    for (int i = 0; i < k; ++i)
-        if (V[i] != fld(Vout, i, c)) RCHK(field_copy(h, fld(Vout, i, c), V[i]));
+    LobBlock b;
+    RCHK(lob_block(h, Vout, k, 0, b));
+    RCHK(block_ortho(h, b.V.data(), k, nullptr, 0, -1.0, b.keepV())); // :262-267
+    for (int iter = 0; iter < maxiter; ++iter) RCHK(lobpcg_sweep(h, phi, b, ritz));
+    RCHK(lob_copy_out(h, b, Vout));
     HIPCHK(hipStreamSynchronize(c.stream));
     return spm_launch_status("spm_lobpcg");
+}
+
+// hiosd_lobpcg_sieh (HISD.cpp:86-485): high-index saddle dynamics of x with the
+// k unstable directions V, one LOBPCG sweep per step at the new x.
+//   f = ngrad_cam(x); tmpp = 2 sum_i <f, V_i> V_i; g = f - tmpp;
+//   bta = |<Dx, Dg>| / <Dg, Dg> (Barzilai-Borwein, Dg = g - g_prev, Dx = x - x_prev),
+//         clamped to [betau dt, betat dt];
+//   x <- the semi-implicit step with force f - tmpp (SOpHisdStep: real part of the spectrum);
+//   then the sweep, f = ngrad_cam(x), stop when nrmp(f) < epsf or after maxiter steps.
+// The first Dx is f - 2 tmpp + dt (the reference adds the constant dt, :185).
+int hisd_run(spm_ctx *h, void *x, void *Vout, int k, double dt, double betat, double betau, double epsf,
+             int maxiter, int *iters, double *res_out)
+{
+    Ctx &c = h->c;
+    LobBlock b;
+    RCHK(lob_block(h, Vout, k, 7, b));
+    void *f = fld(h->ws, 6 * k + 0, c), *g = fld(h->ws, 6 * k + 1, c), *gp = fld(h->ws, 6 * k + 2, c);
+    void *Dx = fld(h->ws, 6 * k + 3, c), *Dg = fld(h->ws, 6 * k + 4, c), *tmpp = fld(h->ws, 6 * k + 5, c);
+    void *xp = fld(h->ws, 6 * k + 6, c);
+    RCHK(block_ortho(h, b.V.data(), k, nullptr, 0, -1.0, b.keepV())); // :151-156
+    // tmpp = s sum_i <f, V_i> V_i  (PV_Cplx, BasicOperators.cpp:360-371)
+    auto pv = [&](double s) -> int {
+        std::vector<double> a(k);
+        const void *fv[1] = {f};
+        RCHK(block_gram(h, cv(b.V).data(), k, fv, 1, a.data()));
+        for (double &ai : a) ai *= s;
+        void *out[1] = {tmpp};
+        return block_combine(h, cv(b.V).data(), k, out, 1, a.data(), nullptr);
+    };
+    // out = a1 in1 + a2 in2 (+ shift)
+    auto lin2 = [&](void *out, const void *in1, double a1, const void *in2, double a2, double shift) -> int {
+        const void *in[2] = {in1, in2};
+        void *o[1] = {out};
+        const double coef[2] = {a1, a2};
+        return block_combine(h, in, 2, o, 1, coef, shift != 0.0 ? &shift : nullptr);
+    };
+    c.g->ngrad(c, x, f);
+    RCHK(pv(1.0));
+    RCHK(lin2(Dx, f, 1.0, tmpp, -2.0, dt));
+    double res = 0.0;
+    int iter = 0;
+    while (iter < maxiter) {
+        RCHK(pv(2.0));
+        RCHK(lin2(g, f, 1.0, tmpp, -1.0, 0.0));
+        RCHK(lin2(Dg, g, 1.0, gp, -1.0, 0.0));
+        RCHK(field_copy(h, gp, g));
+        double dd[2];
+        {
+            const void *X[2] = {Dx, Dg}, *Yv[1] = {Dg};
+            RCHK(block_gram(h, X, 2, Yv, 1, dd));
+        }
+        double bta = std::fabs(dd[0] / dd[1]);
+        bta = std::min(bta, betat * dt);
+        bta = std::max(bta, betau * dt);
+        RCHK(field_copy(h, xp, x));
+        c.g->hisd_step(c, x, tmpp, bta);
+        RCHK(lin2(Dx, x, 1.0, xp, -1.0, 0.0));
+        RCHK(lobpcg_sweep(h, x, b, nullptr));
+        c.g->ngrad(c, x, f);
+        double g2;
+        RCHK(spm_read_scalars(h, c.scal + SC_GN2, 1, &g2));
+        res = std::sqrt(g2);
+        iter = iter + 1;
+        if (res < epsf) break;
+    }
+    RCHK(lob_copy_out(h, b, Vout));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (iters) *iters = iter;
+    if (res_out) *res_out = res;
+    return spm_launch_status("spm_hisd");
 }
 
 } // namespace spm
@@ -244,6 +355,18 @@ int spm_lobpcg(spm_ctx *h, const void *phi, void *V, int k, int maxiter, double 
     if (k < 1 || k > 8) return spm_fail(SPM_EINVAL, "spm_lobpcg: block size k must be 1..8");
     if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_lobpcg: maxiter >= 0");
     return lobpcg_run(h, phi, V, k, maxiter, ritz);
+}
+
+int spm_hisd(spm_ctx *h, void *x, void *V, int k, double dt, double betat, double betau, double epsf, int maxiter,
+             int *iters, double *res)
+{
+    CHECK_CTX(h);
+    if (!x || !V || x == V) return spm_fail(SPM_EINVAL, "spm_hisd: bad fields");
+    if (k < 1 || k > 8) return spm_fail(SPM_EINVAL, "spm_hisd: index k must be 1..8");
+    if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_hisd: maxiter >= 0");
+    if (!(dt > 0) || !(betau > 0) || !(betat >= betau))
+        return spm_fail(SPM_EINVAL, "spm_hisd: need dt > 0 and 0 < betau <= betat");
+    return hisd_run(h, x, V, k, dt, betat, betau, epsf, maxiter, iters, res);
 }
 
 int spm_gram(spm_ctx *h, const void *X, int ma, const void *Y, int mb, double *out)

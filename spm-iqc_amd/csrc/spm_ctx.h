@@ -96,6 +96,8 @@ struct GridIface {
     // (one per block), the <f, d> sums of both replicas (OpFvOut, 7 per block)
     // into pf; returns the block count
     virtual long spm_tangent(Ctx &c, const void *p0, const void *p1, void *d, double *pt, double *pf) = 0;
+    // hiosd_lobpcg_sieh's step of x (HISD.cpp:215-268) with the reflected force part tmpp
+    virtual void hisd_step(Ctx &c, void *x, const void *tmpp, double bta) = 0;
     // force, then scal[SC_GN2] = nrmp(ngrad_cam(x))^2 of the updated x (no gradient field)
     virtual void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef,
                              ForceSpec *fs = nullptr) = 0;

@@ -191,3 +191,30 @@ def test_spring_pairing_matches_oracle(spm, po, seeds, torch, N):
     assert abs(res - res_o) <= 1e-10 * res_o
     assert np.abs(a - P0.real).max() < 1e-11
     assert np.abs(b - P1.real).max() < 1e-11
+
+
+@pytest.mark.parametrize("N,k", [(16, 1), (16, 2), (64, 1)])
+def test_hisd_matches_oracle(spm, po, seeds, torch, N, k):
+    """hiosd_lobpcg_sieh (HISD.cpp:86-485) from the same field and block, five steps
+    without the stopping test: the field, the residual and the span of V, by the
+    axis-by-axis passes (16^3) and the two-pass ones (64^3)."""
+    print("[case] hisd vs oracle N=%d k=%d" % (N, k), flush=True)
+    steps = 5
+    x0 = _relaxed(po, seeds, N, 30, 12) + 0.01 * seeds.gaussian(N, 1.0, 13)
+    o = po.Oracle(N)
+    Vo = o.lobpcg_initialization(_start(k, N, 14))
+    Vstart = Vo.real.copy()
+    xo = (x0 + 0j).copy()
+    it_o, res_o = o.hiosd_lobpcg_sieh(xo, Vo, maxiter=steps, epsf=-1.0)
+    with spm.Grid(N, 0) as g:
+        x = torch.as_tensor(x0).cuda()
+        V = torch.as_tensor(Vstart).cuda()
+        it, res = g.hisd(x, V, maxiter=steps, epsf=-1.0)
+        x, V = _np(x), _np(V)
+    assert it == it_o == steps
+    assert abs(res - res_o) <= 1e-8 * res_o
+    assert np.abs(x - xo.real).max() < 1e-10
+    A = V.reshape(k, -1).T / N ** 1.5
+    B = Vo.real.reshape(k, -1).T / N ** 1.5
+    s = np.linalg.svd(A.T @ B, compute_uv=False)
+    assert s.min() > 1 - 1e-8
