@@ -92,6 +92,10 @@ void precond_camnew(fftw_complex *w);
 void hiosd_lobpcg_initialization(Option *option, int ki);
 void hiosd_lobpcg_inip(fftw_complex *x, Option *option);
 void copy_option_V(Option *option1, Option *option2, int k);
+/* HISD.cpp:13-485 (include/HISD.h) */
+void critical_point_HISD();
+void critical_point_HISD_v();
+void hiosd_lobpcg_sieh(fftw_complex *x, Option *option);
 void copy_option_V(fftw_complex *v, Option *option2, int k);
 void copy_option_V(fftw_complex **V, Option *option2, int k);
 

@@ -432,6 +432,62 @@ void hiosd_lobpcg_inip(fftw_complex *x, Option *option)
     }
 }
 
+// HISD.cpp:86-485, on the device through spm_hisd.  As the reference, it
+// leaves x in u_R_local and its spectrum in u_F_local (:446-448).
+void hiosd_lobpcg_sieh(fftw_complex *x, Option *option)
+{
+    const int k = option->k;
+    to_real(x, g_re[0]);
+    double *blk = block(k);
+    for (int i = 0; i < k; i++) to_real(option->V[i], blk + i * R());
+    int it = 0;
+    double res = 0.0;
+    CK(spm_hisd(ctx(), g_re[0], blk, k, option->dt, option->betat, option->betau, option->epsf, option->maxiter, &it,
+                &res));
+    to_cplx(g_re[0], x);
+    for (int i = 0; i < k; i++) to_cplx(blk + i * R(), option->V[i]);
+    if (x != u_R_local) Cpy(x, u_R_local, CplxDofs_local);
+    my_fft(u_R_local, u_F_local);
+    if (myid == 0) std::printf("\n %d \t res = %.3e \n", it, res);
+}
+
+// HISD.cpp:13-30: index-1 saddle search from u_R_local + 0.01 V_1 of option1
+void critical_point_HISD()
+{
+    const int k = 1;
+    hiosd_lobpcg_initialization(option2, k);
+    option2->dt = 0.8;
+    option2->betat = 8.0;
+    option2->betau = 0.02;
+    option2->epsf = 1e-7;
+    option2->maxiter = 10000;
+    option2->outputp = 100;
+    option2->outputd = 1000;
+    FuncsLinear2Cplx(u_R_local, CplxDofs_local, 1.0, u_R_local, 0.01, option1->V[k - 1]);
+    copy_option_V(option1, option2, k);
+    hiosd_lobpcg_sieh(u_R_local, option2);
+}
+
+// HISD.cpp:32-82: the same with betau = 0.6.  The reference also reads the FCC
+// seed into a field it then leaves unused; the read (and its file) is kept.
+void critical_point_HISD_v()
+{
+    const int k = 1;
+    hiosd_lobpcg_initialization(option2, k);
+    option2->dt = 0.8;
+    option2->betat = 8.0;
+    option2->betau = 0.6;
+    option2->epsf = 1e-7;
+    option2->maxiter = 10000;
+    option2->outputp = 100;
+    option2->outputd = 1000;
+    std::vector<fftw_complex> v((size_t)CplxDofs);
+    get_initial_value(v.data(), 2, 0.0);
+    FuncsLinear2Cplx(u_R_local, CplxDofs_local, 1.0, u_R_local, 0.01, option1->V[k - 1]);
+    copy_option_V(option1, option2, k);
+    hiosd_lobpcg_sieh(u_R_local, option2);
+}
+
 void copy_option_V(Option *o1, Option *o2, int k)
 {
     for (int i = 0; i < k; i++) {

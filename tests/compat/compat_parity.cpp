@@ -165,11 +165,43 @@ int main(int argc, char **argv)
     double lam_ref = 1e300;
     for (size_t l = 1; l < R; ++l) lam_ref = std::fmin(lam_ref, epmckt2_local[l] - Tau);
     Memfree_Option(opt);
+    double e_hx, e_hv1, e_hres;
+    {   // hiosd_lobpcg_sieh, 5 steps from phi with one unit mode v
+        field vu = v;
+        double nv = 0;
+        for (size_t l = 0; l < R; ++l) nv += vu[l].re * vu[l].re;
+        nv = std::sqrt(nv / R);
+        for (size_t l = 0; l < R; ++l) vu[l].re /= nv;
+        Option *h = (Option *)std::malloc(sizeof(Option));
+        hiosd_lobpcg_initialization(h, 1);
+        h->dt = 0.8;
+        h->betat = 8.0;
+        h->betau = 0.02;
+        h->epsf = 1e-30;
+        h->maxiter = 5;
+        up(h->V[0], vu);
+        up(u_R_local, phi);
+        hiosd_lobpcg_sieh(u_R_local, h);
+        field x = phi, vr = vu;
+        double res = 0;
+        lpo_hiosd_lobpcg_sieh(o, FP(x), FP(vr), 1, 0.8, 8.0, 0.02, 1e-30, 5, &res);
+        e_hx = maxdiff_re(down(u_R_local), x);
+        const field vg = down(h->V[0]);
+        double dot = 0, ng = 0, nr = 0;
+        for (size_t l = 0; l < R; ++l) dot += vg[l].re * vr[l].re, ng += vg[l].re * vg[l].re, nr += vr[l].re * vr[l].re;
+        e_hv1 = 1.0 - std::fabs(dot) / std::sqrt(ng * nr);
+        // the layer leaves x in u_R_local and its spectrum in u_F_local; its
+        // residual is nrmp(ngrad_cam(x)), recomputed here from the result
+        ngrad_cam(u_R_local, fftw_Ctmp_local);
+        e_hres = std::fabs(nrmp(fftw_Ctmp_local) - res) / res;
+        Memfree_Option(h);
+    }
     std::printf("{\"fft\": %.3e, \"ifft\": %.3e, \"energy_rel\": %.3e, \"ngrad\": %.3e, \"hv\": %.3e, "
                 "\"precond\": %.3e, \"inpfp\": %.3e, \"gradflow\": %.3e, \"tv\": %.3e, \"natural\": %.3e, "
-                "\"minmax\": %.3e, \"spring\": %.3e, \"file\": %.3e, \"lobpcg_lowest\": %.3e}\n",
+                "\"minmax\": %.3e, \"spring\": %.3e, \"file\": %.3e, \"lobpcg_lowest\": %.3e, "
+                "\"hisd_x\": %.3e, \"hisd_v\": %.3e, \"hisd_res_rel\": %.3e}\n",
                 e_fft, e_ifft, e_E, e_grad, e_hv, e_pc, e_inp, e_gf, e_tv, e_nat, e_mm, e_sp, e_file,
-                std::fabs(lam - lam_ref));
+                std::fabs(lam - lam_ref), e_hx, e_hv1, e_hres);
     lpo_destroy(o);
     Memfree_global_kspace();
     Memfree_global_data();

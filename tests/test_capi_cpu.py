@@ -66,3 +66,34 @@ def test_step_passes_validated_before_any_device_work(spm):
         p = spm._params(N, spm.FP64, 26.0, 1.0, -0.0061, 0.30, 0.3, 0, passes)
         assert lib.spm_create(ctypes.byref(p), ctypes.byref(h)) == want
         assert not h.value
+
+
+def test_compat_library_defines_reference_api():
+    """every function include/compat/spm_reference_api.h declares (the
+    reference's signatures) is defined in the compat library"""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    so = os.path.join(ROOT, "spm-iqc_amd", "lib", "libspm_iqc_compat.so")
+    if not os.path.exists(so) or shutil.which("nm") is None:
+        pytest.skip("compat library not built or nm missing")
+    src = open(os.path.join(ROOT, "include", "compat", "spm_reference_api.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    src = re.sub(r"#[^\n]*", "", src)
+    src = re.sub(r"struct\s+\w+\s*\{.*?\};", "", src, flags=re.S)
+    names = set()
+    for stmt in src.split(";"):
+        stmt = " ".join(stmt.split())
+        if stmt.startswith(("extern", "typedef")):
+            continue
+        m = re.match(r"[\w\s\*]+?\b(\w+)\s*\(", stmt)
+        if m:
+            names.add(m.group(1))
+    assert len(names) >= 40, sorted(names)
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    defined = set(re.findall(r" T (\w+)\(", out))
+    missing = sorted(names - defined)
+    assert not missing, missing

@@ -37,6 +37,10 @@ def test_compat_functions_match_oracle(N):
     # near-degenerate eigenvalues and 12 x 21 iterations of block 2 do not
     # resolve it to 1e-9; N = 16 has a separated lowest eigenvalue
     assert res["lobpcg_lowest"] < (1e-9 if N == 16 else 1e-6)
+    # hiosd_lobpcg_sieh: 5 HiSD steps (x-step + one LOBPCG sweep each)
+    assert res["hisd_x"] < 1e-10
+    assert res["hisd_v"] < 1e-8
+    assert res["hisd_res_rel"] < 1e-8
 
 
 def test_stage1_driver(seeds, tmp_path):
