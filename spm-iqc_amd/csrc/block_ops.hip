@@ -1,6 +1,7 @@
 // block_ops.hip -- Gram matrices on the matrix cores, linear combinations and
 // means of small blocks of fields, and the CholQR2 orthonormalisation built
 // on them.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -115,58 +116,130 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
     }
 }
 
-// out[e] = scale * sum over blocks of part[b][e]: 64 entries per block, the
-// blocks b = g (mod 4) summed by thread group g in a fixed order, then the 4
-// group sums combined in a fixed order (the result does not depend on timing).
-__global__ void __launch_bounds__(256) k_gram_reduce(const double *__restrict__ part, int nblk, double scale,
-                                                    double *__restrict__ out)
+// The Gram of blocks of at most SMALL_M fields, where an MFMA tile would keep
+// only ma + mb of its 16 rows loading (a latency-bound stream).  Each thread
+// sums 4 consecutive points per field and iteration into ma x mb FP64
+// accumulators; the wave sums by a fixed butterfly, the 4 waves in a fixed
+// order, and the block writes its partials in k_gram's layout (entry a*32 + b),
+// so k_gram_reduce finishes both kernels.  SAME: Y is X (a block against
+// itself), read once.
+constexpr int SMALL_M = 4;
+template <typename T, bool SAME>
+__global__ void __launch_bounds__(256) k_gram_small(FieldPtrs X, int ma, FieldPtrs Y, int mb, long R,
+                                                   double *__restrict__ part)
 {
-    __shared__ double sh[4][64];
+    __shared__ double sh[4][SMALL_M * SMALL_M];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc[SMALL_M][SMALL_M];
+#pragma unroll
+    for (int a = 0; a < SMALL_M; ++a)
+#pragma unroll
+        for (int b = 0; b < SMALL_M; ++b) acc[a][b] = 0.0;
+    auto load4 = [&](const void *f, long r, double (&v)[4]) {
+        if constexpr (sizeof(T) == 8) {
+            const double2 u = *((const double2 *)f + r / 2), w = *((const double2 *)f + r / 2 + 1);
+            v[0] = u.x; v[1] = u.y; v[2] = w.x; v[3] = w.y;
+        } else {
+            const float4 u = *((const float4 *)f + r / 4);
+            v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+        }
+    };
+    const long stride = (long)gridDim.x * blockDim.x * 4;
+#pragma unroll 2
+    for (long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < R; r += stride) {
+        double x[SMALL_M][4], y[SMALL_M][4];
+#pragma unroll
+        for (int a = 0; a < SMALL_M; ++a)
+            if (a < ma) load4(X.p[a], r, x[a]);
+#pragma unroll
+        for (int b = 0; b < SMALL_M; ++b)
+            if (!SAME && b < mb) load4(Y.p[b], r, y[b]);
+#pragma unroll
+        for (int a = 0; a < SMALL_M; ++a)
+#pragma unroll
+            for (int b = 0; b < SMALL_M; ++b)
+                if (a < ma && b < mb) {
+                    const double(&yb)[4] = SAME ? x[b] : y[b];
+                    acc[a][b] += ((x[a][0] * yb[0] + x[a][1] * yb[1]) + x[a][2] * yb[2]) + x[a][3] * yb[3];
+                }
+    }
+#pragma unroll
+    for (int a = 0; a < SMALL_M; ++a)
+#pragma unroll
+        for (int b = 0; b < SMALL_M; ++b) {
+            if (a >= ma || b >= mb) continue;
+            double v = acc[a][b];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) sh[wid][a * SMALL_M + b] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < SMALL_M * SMALL_M) {
+        const int a = threadIdx.x / SMALL_M, b = threadIdx.x % SMALL_M;
+        if (a < ma && b < mb)
+            part[(long)blockIdx.x * GRAM_TILE * GRAM_TILE + a * GRAM_TILE + b] =
+                ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
+    }
+}
+
+// out[e] = scale * sum over blocks of part[b][e], for the 64 entries of this
+// block: thread group g (16 groups of 64) sums the blocks b = g (mod 16) in
+// order, then the 16 group sums combine by a fixed pairwise tree, so the
+// result does not depend on timing.
+__global__ void __launch_bounds__(1024) k_gram_reduce(const double *__restrict__ part, int nblk, double scale,
+                                                     double *__restrict__ out)
+{
+    __shared__ double sh[16][64];
     const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int e = blockIdx.x * 64 + el;
     double s = 0.0;
 #pragma unroll 8
-    for (int b = g; b < nblk; b += 4) s += part[(long)b * GRAM_TILE * GRAM_TILE + e];
+    for (int b = g; b < nblk; b += 16) s += part[(long)b * GRAM_TILE * GRAM_TILE + e];
     sh[g][el] = s;
     __syncthreads();
-    if (g == 0) out[e] = ((sh[0][el] + sh[1][el]) + (sh[2][el] + sh[3][el])) * scale;
+#pragma unroll
+    for (int w = 8; w > 0; w >>= 1) {
+        if (g < w) sh[g][el] += sh[g + w][el];
+        __syncthreads();
+    }
+    if (g == 0) out[e] = sh[0][el] * scale;
 }
 
-// Operands of k_combine, uploaded to the device with its coefficients: field
-// pointers and coefficients are then read with uniform (scalar) loads.
+constexpr int COMBINE_NO = 8; // outputs per k_combine launch
+
+// Operands of one k_combine launch, passed by value (2.4 KB of kernel
+// arguments, read with uniform scalar loads): no upload, so a combine queues
+// behind earlier work without draining the stream.
 struct CombineArgs {
     const void *in[BLK_MAX];
-    void *out[BLK_MAX];
+    void *out[COMBINE_NO];
+    double coef[BLK_MAX * COMBINE_NO]; // coef[a * COMBINE_NO + o]
+    double shift[COMBINE_NO];
 };
 
-// out_{i0+o} = sum_a coef[a*nout + i0+o] in_a + shift[i0+o], o < NO: four
-// consecutive points per thread, all inputs read once per point (so an
-// output may alias an input).
+// out_o = sum_a coef[a][o] in_a + shift[o], o < NO: four consecutive points
+// per thread, all inputs read once per point (so an output may alias an input).
 template <typename T, int NO>
-__global__ void __launch_bounds__(256) k_combine(const CombineArgs *__restrict__ args, int nin, int nout, int i0,
-                                                const double *__restrict__ coef, const double *__restrict__ shift,
-                                                long R4)
+__global__ void __launch_bounds__(256) k_combine(const CombineArgs args, int nin, long R4)
 {
     for (long r4 = (long)blockIdx.x * blockDim.x + threadIdx.x; r4 < R4; r4 += (long)gridDim.x * blockDim.x) {
         double acc[NO][4];
 #pragma unroll
-        for (int o = 0; o < NO; ++o) {
-            const double sh = shift ? shift[i0 + o] : 0.0;
+        for (int o = 0; o < NO; ++o)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc[o][k] = sh;
-        }
+            for (int k = 0; k < 4; ++k) acc[o][k] = args.shift[o];
         for (int a = 0; a < nin; ++a) {
             double x[4];
             if constexpr (sizeof(T) == 4) {
-                const float4 u = ((const float4 *)args->in[a])[r4];
+                const float4 u = ((const float4 *)args.in[a])[r4];
                 x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w;
             } else {
-                const double2 u = ((const double2 *)args->in[a])[2 * r4], w = ((const double2 *)args->in[a])[2 * r4 + 1];
+                const double2 u = ((const double2 *)args.in[a])[2 * r4], w = ((const double2 *)args.in[a])[2 * r4 + 1];
                 x[0] = u.x; x[1] = u.y; x[2] = w.x; x[3] = w.y;
             }
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
-                const double cf = coef[a * nout + i0 + o];
+                const double cf = args.coef[a * COMBINE_NO + o];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) acc[o][k] += cf * x[k];
             }
@@ -174,11 +247,11 @@ __global__ void __launch_bounds__(256) k_combine(const CombineArgs *__restrict__
 #pragma unroll
         for (int o = 0; o < NO; ++o) {
             if constexpr (sizeof(T) == 4) {
-                ((float4 *)args->out[i0 + o])[r4] = float4{(float)acc[o][0], (float)acc[o][1], (float)acc[o][2],
-                                                           (float)acc[o][3]};
+                ((float4 *)args.out[o])[r4] = float4{(float)acc[o][0], (float)acc[o][1], (float)acc[o][2],
+                                                     (float)acc[o][3]};
             } else {
-                ((double2 *)args->out[i0 + o])[2 * r4] = double2{acc[o][0], acc[o][1]};
-                ((double2 *)args->out[i0 + o])[2 * r4 + 1] = double2{acc[o][2], acc[o][3]};
+                ((double2 *)args.out[o])[2 * r4] = double2{acc[o][0], acc[o][1]};
+                ((double2 *)args.out[o])[2 * r4 + 1] = double2{acc[o][2], acc[o][3]};
             }
         }
     }
@@ -241,7 +314,19 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     double *part = gram_part(h);
 #define GRAM_LAUNCH(T, A, B)                                                                                   \
     hipLaunchKernelGGL((k_gram<T, A, B>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
-    if (c.prec == SPM_FP64) {
+#define GRAM_SMALL(T, SAME)                                                                                    \
+    hipLaunchKernelGGL((k_gram_small<T, SAME>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
+    // R = N^3 with 4 | N: a multiple of 64, so every thread's 4-point group is whole
+    if (ma <= SMALL_M && mb <= SMALL_M) {
+        const bool same = ma == mb && std::equal(X, X + ma, Y);
+        if (c.prec == SPM_FP64) {
+            if (same) GRAM_SMALL(double, true);
+            else GRAM_SMALL(double, false);
+        } else {
+            if (same) GRAM_SMALL(float, true);
+            else GRAM_SMALL(float, false);
+        }
+    } else if (c.prec == SPM_FP64) {
         if (ta == 1 && tb == 1) GRAM_LAUNCH(double, 1, 1);
         else if (ta == 1) GRAM_LAUNCH(double, 1, 2);
         else if (tb == 1) GRAM_LAUNCH(double, 2, 1);
@@ -253,12 +338,14 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
         else GRAM_LAUNCH(float, 2, 2);
     }
 #undef GRAM_LAUNCH
+#undef GRAM_SMALL
     double *dout = small_dev(h);
-    hipLaunchKernelGGL(k_gram_reduce, dim3(GRAM_TILE * GRAM_TILE / 64), dim3(256), 0, c.stream, part, nblk,
+    // rows 0 .. ma-1 of the 32-wide partials: (ma * 32 + 63) / 64 groups of 64 entries
+    hipLaunchKernelGGL(k_gram_reduce, dim3((ma * GRAM_TILE + 63) / 64), dim3(1024), 0, c.stream, part, nblk,
                        1.0 / (double)c.R, dout);
     RCHK(spm_launch_status("block_gram"));
     std::vector<double> full(GRAM_TILE * GRAM_TILE);
-    HIPCHK(hipMemcpyAsync(full.data(), dout, sizeof(double) * full.size(), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(full.data(), dout, sizeof(double) * ma * GRAM_TILE, hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     for (int a = 0; a < ma; ++a)
         for (int b = 0; b < mb; ++b) out[a * mb + b] = full[a * GRAM_TILE + b];
@@ -270,33 +357,23 @@ int block_combine(spm_ctx *h, const void *const *in, int nin, void *const *out, 
 {
     Ctx &c = h->c;
     if (nin < 1 || nout < 1 || nin > BLK_MAX || nout > BLK_MAX) return spm_fail(SPM_EINVAL, "block_combine");
-    constexpr int NO = 8; // outputs per launch
-    if (nout > NO)        // later launches would read inputs that earlier ones overwrote
+    constexpr int NO = COMBINE_NO;
+    if (nout > NO) // later launches would read inputs that earlier ones overwrote
         for (int a = 0; a < nin; ++a)
             for (int i = 0; i < nout; ++i)
                 if (in[a] == out[i]) return spm_fail(SPM_EINVAL, "block_combine: in place with more than 8 outputs");
-    RCHK(ensure_bscr(h));
-    // operands and coefficients: uploaded after the stream has drained (the
-    // scratch may be in use by queued work)
-    double *dsm = small_dev(h) + GRAM_TILE * GRAM_TILE;
-    CombineArgs ca{};
-    for (int a = 0; a < nin; ++a) ca.in[a] = in[a];
-    for (int i = 0; i < nout; ++i) ca.out[i] = out[i];
-    const size_t nargs = (sizeof(CombineArgs) + sizeof(double) - 1) / sizeof(double);
-    std::vector<double> buf(nargs + nin * nout + nout, 0.0);
-    std::memcpy(buf.data(), &ca, sizeof(ca));
-    std::memcpy(buf.data() + nargs, coef, sizeof(double) * nin * nout);
-    if (shift) std::memcpy(buf.data() + nargs + nin * nout, shift, sizeof(double) * nout);
-    HIPCHK(hipStreamSynchronize(c.stream));
-    HIPCHK(hipMemcpy(dsm, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
-    const CombineArgs *dargs = (const CombineArgs *)dsm;
-    const double *dc = dsm + nargs, *ds = shift ? dsm + nargs + nin * nout : nullptr;
     const long R4 = c.R / 4; // N^3, N >= 4: a multiple of 4
     const int nb = (int)std::min<long>(2048, (R4 + 255) / 256);
     for (int i0 = 0; i0 < nout; i0 += NO) {
         const int no = std::min(NO, nout - i0);
-#define COMBINE_LAUNCH(T, K)                                                                                   \
-    hipLaunchKernelGGL((k_combine<T, K>), dim3(nb), dim3(256), 0, c.stream, dargs, nin, nout, i0, dc, ds, R4)
+        CombineArgs ca{};
+        for (int a = 0; a < nin; ++a) ca.in[a] = in[a];
+        for (int o = 0; o < no; ++o) {
+            ca.out[o] = out[i0 + o];
+            ca.shift[o] = shift ? shift[i0 + o] : 0.0;
+            for (int a = 0; a < nin; ++a) ca.coef[a * NO + o] = coef[a * nout + i0 + o];
+        }
+#define COMBINE_LAUNCH(T, K) hipLaunchKernelGGL((k_combine<T, K>), dim3(nb), dim3(256), 0, c.stream, ca, nin, R4)
 #define COMBINE_BY_NO(T)                                                                                       \
     switch (no) {                                                                                              \
     case 1: COMBINE_LAUNCH(T, 1); break;                                                                       \

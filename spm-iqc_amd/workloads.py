@@ -4,10 +4,12 @@ command per kernel family, printing its own timing.
     python spm-iqc_amd/workloads.py gradflow [--precision fp64|fp32] [--steps 50] [--shape -1] [--step-passes 0]
     python spm-iqc_amd/workloads.py lobpcg   [--precision ...] [--iters 5]
     python spm-iqc_amd/workloads.py spm      [--precision ...] [--iters 20]
+    python spm-iqc_amd/workloads.py hisd     [--precision ...] [--iters 20]
 
 gradflow: get_min_u_F iterations on the icosahedral nucleus seed (LPsrc.cpp:40-117).
 lobpcg:   hiosd_lobpcg_inip iterations, block 8, on the same field (LOBPCG.cpp:207-487).
 spm:      Spring_Pairing iterations from phi -/+ 0.01 v (SPM.cpp:170-263).
+hisd:     hiosd_lobpcg_sieh steps, k = 1, from phi + 0.01 v (HISD.cpp:13-30, 86-485).
 """
 from __future__ import annotations
 
@@ -90,9 +92,27 @@ def saddle(N, prec, iters, shape, passes=0):
     print("spm N %d prec %d: %d iterations, %.2f ms/iteration, res %.3e" % (N, prec, it, dt / max(it, 1) * 1e3, res))
 
 
+def hisd(N, prec, iters, shape, passes=0):
+    torch = spm._torch()
+    shape = _shape(N, prec, shape)
+    with spm.Grid(N, prec, shape=shape, step_passes=passes) as g:
+        phi = _field(torch, g, N)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(2)
+        start = (torch.rand((1, N, N, N), generator=gen, dtype=torch.float64, device="cuda") * 2 - 1).to(g.dtype)
+        V = g.lobpcg_init(start)
+        x = phi + 0.01 * V[0]
+        g.hisd(x, V, epsf=0.0, maxiter=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, res = g.hisd(x, V, epsf=0.0, maxiter=iters)
+        dt = time.perf_counter() - t0
+    print("hisd N %d prec %d: %d steps, %.2f ms/step, res %.3e" % (N, prec, it, dt / max(it, 1) * 1e3, res))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("workload", choices=("gradflow", "lobpcg", "spm"))
+    ap.add_argument("workload", choices=("gradflow", "lobpcg", "spm", "hisd"))
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--precision", choices=("fp64", "fp32", "both"), default="both")
     ap.add_argument("--steps", type=int, default=50)
@@ -109,8 +129,10 @@ def main(argv=None):
             gradflow(a.N, p, a.steps, want[p], a.step_passes)
         elif a.workload == "lobpcg":
             lobpcg(a.N, p, a.iters or 5, want[p], a.step_passes)
-        else:
+        elif a.workload == "spm":
             saddle(a.N, p, a.iters or 20, want[p], a.step_passes)
+        else:
+            hisd(a.N, p, a.iters or 20, want[p], a.step_passes)
 
 
 if __name__ == "__main__":

@@ -35,7 +35,7 @@ def _relaxed(po, seeds, N, steps, seed=0):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("ma,mb", [(1, 1), (3, 5), (16, 16), (17, 9), (24, 24), (32, 32)])
+@pytest.mark.parametrize("ma,mb", [(1, 1), (2, 4), (4, 3), (3, 5), (16, 16), (17, 9), (24, 24), (32, 32)])
 def test_gram_matrix_cores(spm, torch, prec, ma, mb):
     """The MFMA Gram equals the FP64 inner products inpfp of every pair."""
     N = 16
@@ -49,6 +49,22 @@ def test_gram_matrix_cores(spm, torch, prec, ma, mb):
         Xs, Ys = _np(Xt).astype(np.float64), _np(Yt).astype(np.float64)
     ref = np.einsum("aijk,bijk->ab", Xs, Ys) / N ** 3
     assert np.abs(out - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("N,m", [(16, 1), (64, 3), (64, 4)])
+def test_gram_small_blocks_self(spm, torch, prec, N, m):
+    """A block of at most 4 fields against itself (the streaming Gram, Y read
+    as X) equals the FP64 inner products, and is symmetric."""
+    rng = np.random.default_rng(N + m)
+    X = rng.standard_normal((m, N, N, N))
+    with spm.Grid(N, prec) as g:
+        Xt = torch.as_tensor(X).to(g.dtype).cuda()
+        out = g.gram(Xt, Xt)
+        Xs = _np(Xt).astype(np.float64)
+    ref = np.einsum("aijk,bijk->ab", Xs, Xs) / N ** 3
+    assert np.abs(out - ref).max() < 1e-13
+    assert np.array_equal(out, out.T)
 
 
 def test_lobpcg_init_matches_oracle(spm, po, torch):
