@@ -20,10 +20,13 @@ constexpr int GRAM_TILE = 32;     // the Gram is computed as a padded 32 x 32 bl
 // D: lane l holds D[l/16 + 4v][l%16], checked on gfx950).  The K index of a chunk is permuted so
 // that lane l reads 4 consecutive points r0 + 4(l/16) .. +3 of its field in
 // one 32-byte (FP64) or 16-byte (FP32) load; the sum over K is unchanged.
-template <typename T, int TA, int TB>
+// SAME (Y is X, TA = TB): A and B of a tile pair take the same K mapping, so
+// lane l's B values of tile t are its A values of tile t and Y is not read.
+template <typename T, int TA, int TB, bool SAME>
 __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, int mb, long R,
                                              double *__restrict__ part)
 {
+    static_assert(!SAME || TA == TB, "k_gram: a block against itself has square tiles");
     __shared__ double sh[4][TA * TB * 4][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int row = lane & 15, kq = lane >> 4;
@@ -36,7 +39,8 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
 #pragma unroll
     for (int ta = 0; ta < TA; ++ta) xa[ta] = (ta * 16 + row < ma) ? (const T *)X.p[ta * 16 + row] : nullptr;
 #pragma unroll
-    for (int tb = 0; tb < TB; ++tb) yb[tb] = (tb * 16 + row < mb) ? (const T *)Y.p[tb * 16 + row] : nullptr;
+    for (int tb = 0; tb < TB; ++tb)
+        yb[tb] = (!SAME && tb * 16 + row < mb) ? (const T *)Y.p[tb * 16 + row] : nullptr;
     const long nchunk = R / 16;
     // lane data of chunk q: 4 consecutive points r0 + 4 (l / 16) .. +3 of its row's field
     auto load = [&](long q, double (&a)[TA][4], double (&b)[TB][4]) {
@@ -57,6 +61,7 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
         }
 #pragma unroll
         for (int tb = 0; tb < TB; ++tb) {
+            if constexpr (SAME) continue;
             if (yb[tb]) {
                 if constexpr (sizeof(T) == 8) {
                     const double2 u = *(const double2 *)(yb[tb] + r0), w = *(const double2 *)(yb[tb] + r0 + 2);
@@ -84,7 +89,8 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
             for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
                 for (int tb = 0; tb < TB; ++tb)
-                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], b[tb][s], acc[ta][tb], 0, 0, 0);
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], SAME ? a[tb][s] : b[tb][s],
+                                                                       acc[ta][tb], 0, 0, 0);
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
@@ -312,13 +318,13 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     const int ta = ma > 16 ? 2 : 1, tb = mb > 16 ? 2 : 1;
     const int nblk = (int)std::min<long>(GRAM_BLOCKS, std::max<long>(1, c.R / 64));
     double *part = gram_part(h);
-#define GRAM_LAUNCH(T, A, B)                                                                                   \
-    hipLaunchKernelGGL((k_gram<T, A, B>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
+#define GRAM_LAUNCH(T, A, B, S)                                                                                \
+    hipLaunchKernelGGL((k_gram<T, A, B, S>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
 #define GRAM_SMALL(T, SAME)                                                                                    \
     hipLaunchKernelGGL((k_gram_small<T, SAME>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
     // R = N^3 with 4 | N: a multiple of 64, so every thread's 4-point group is whole
+    const bool same = ma == mb && std::equal(X, X + ma, Y);
     if (ma <= SMALL_M && mb <= SMALL_M) {
-        const bool same = ma == mb && std::equal(X, X + ma, Y);
         if (c.prec == SPM_FP64) {
             if (same) GRAM_SMALL(double, true);
             else GRAM_SMALL(double, false);
@@ -327,15 +333,21 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
             else GRAM_SMALL(float, false);
         }
     } else if (c.prec == SPM_FP64) {
-        if (ta == 1 && tb == 1) GRAM_LAUNCH(double, 1, 1);
-        else if (ta == 1) GRAM_LAUNCH(double, 1, 2);
-        else if (tb == 1) GRAM_LAUNCH(double, 2, 1);
-        else GRAM_LAUNCH(double, 2, 2);
+        if (ta == 1 && tb == 1) {
+            if (same) GRAM_LAUNCH(double, 1, 1, true);
+            else GRAM_LAUNCH(double, 1, 1, false);
+        } else if (ta == 1) GRAM_LAUNCH(double, 1, 2, false);
+        else if (tb == 1) GRAM_LAUNCH(double, 2, 1, false);
+        else if (same) GRAM_LAUNCH(double, 2, 2, true);
+        else GRAM_LAUNCH(double, 2, 2, false);
     } else {
-        if (ta == 1 && tb == 1) GRAM_LAUNCH(float, 1, 1);
-        else if (ta == 1) GRAM_LAUNCH(float, 1, 2);
-        else if (tb == 1) GRAM_LAUNCH(float, 2, 1);
-        else GRAM_LAUNCH(float, 2, 2);
+        if (ta == 1 && tb == 1) {
+            if (same) GRAM_LAUNCH(float, 1, 1, true);
+            else GRAM_LAUNCH(float, 1, 1, false);
+        } else if (ta == 1) GRAM_LAUNCH(float, 1, 2, false);
+        else if (tb == 1) GRAM_LAUNCH(float, 2, 1, false);
+        else if (same) GRAM_LAUNCH(float, 2, 2, true);
+        else GRAM_LAUNCH(float, 2, 2, false);
     }
 #undef GRAM_LAUNCH
 #undef GRAM_SMALL
