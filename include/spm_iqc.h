@@ -159,6 +159,15 @@ int spm_rayleighq(spm_ctx *c, const void *phi, const void *v, double *out);
  * (HISD.cpp:16-22).  1 <= k <= 8. */
 int spm_hisd(spm_ctx *c, void *x, void *V, int k, double dt, double betat, double betau, double epsf, int maxiter,
              int *iters, double *res);
+/* hiosd_lobpcg_sieh_V0 (GOSD.cpp:123-531): spm_hisd with the k0 fixed
+ * orthonormal fields V0 (the null space, k0 contiguous fields, read only, not
+ * overlapping x or V): the embedded LOBPCG keeps its search directions W, Vp
+ * and the updated V orthogonal to V0 (GOSD.cpp:206, 227, 347).  k0 = 0 is
+ * spm_hisd.  The reference's drivers use k0 = 3, dt 0.8, betat 8, betau 1.3
+ * (0.4 for critical_point_v), epsf 1e-9 (1e-7), maxiter 40000 (20000)
+ * (GOSD.cpp:13-120).  1 <= k <= 8, 2 k + k0 <= 16. */
+int spm_gosd(spm_ctx *c, void *x, void *V, int k, const void *V0, int k0, double dt, double betat, double betau,
+             double epsf, int maxiter, int *iters, double *res);
 
 /* --- SPM */
 /* get_Tv (SPM.cpp:15-32): v = (phi1 - phi0) / nrmp(phi1 - phi0). */

@@ -749,10 +749,30 @@ int lpo_lobpcg_inip(lpo_ctx *c, const lpo_cplx *x, lpo_cplx *Vio, int k, int max
  * x, Vio (k fields) in place; returns the iteration count, *res_out the last
  * nrmp(ngrad_cam(x)).  EigenSolver's complex-eigenvalue exit (:363-372) cannot
  * occur here: the symmetrised Rayleigh-Ritz matrix goes to the Jacobi solver. */
+static int hisd_core(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, const lpo_cplx *V0in, int k0, double dt,
+                     double betat, double betau, double epsf, int maxiter, double *res_out);
+
 int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double dt, double betat, double betau,
                           double epsf, int maxiter, double *res_out)
 {
+    return hisd_core(c, x, Vio, k, NULL, 0, dt, betat, betau, epsf, maxiter, res_out);
+}
+
+/* hiosd_lobpcg_sieh_V0 (GOSD.cpp:123-531): the same dynamics with the sweep's
+ * W, Vp and updated V also orthogonalised against the k0 fixed fields V0
+ * (GOSD.cpp:206, 227, 347), each right after its projection on V (on W for Vp). */
+int lpo_hiosd_lobpcg_sieh_V0(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, const lpo_cplx *V0, int k0, double dt,
+                             double betat, double betau, double epsf, int maxiter, double *res_out)
+{
+    return hisd_core(c, x, Vio, k, V0, k0, dt, betat, betau, epsf, maxiter, res_out);
+}
+
+static int hisd_core(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, const lpo_cplx *V0in, int k0, double dt,
+                     double betat, double betau, double epsf, int maxiter, double *res_out)
+{
     const long R = c->R;
+    lpo_cplx **V0 = (lpo_cplx **)malloc(sizeof(void *) * (k0 > 0 ? k0 : 1));
+    for (int i = 0; i < k0; ++i) V0[i] = (lpo_cplx *)(V0in + (long)i * R); /* read only */
     const double Tau = c->Tau, Gamma = c->Gamma;
     lpo_cplx **V = (lpo_cplx **)malloc(sizeof(void *) * 6 * k);
     lpo_cplx **U = V + k, **W = V + 2 * k, **Y = V + 3 * k, **Vp = V + 4 * k, **Up = V + 5 * k;
@@ -817,8 +837,8 @@ int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double 
             gp[j][1] = 1.0 * g[j][1];
         }
         double bta = fabs(lpo_inpfp(c, Dx, Dg) / lpo_inpfp(c, Dg, Dg)); /* :205-212 */
-        bta = bta < betat * dt ? bta : betat * dt;
-        bta = bta > betau * dt ? bta : betau * dt;
+        bta = betat * dt < bta ? betat * dt : bta; /* std::min, std::max: a NaN step stays NaN */
+        bta = bta < betau * dt ? betau * dt : bta;
         for (long j = 0; j < R; ++j) {
             xp[j][0] = 1.0 * x[j][0];
             xp[j][1] = 1.0 * x[j][1];
@@ -863,6 +883,7 @@ int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double 
             }
             lpo_precond_camnew(c, W[i]);
             orthoi_v2V(c, W[i], V, k);
+            orthoi_v2V(c, W[i], V0, k0);
             orthoi_Vi(c, W, i);
             const double nrmW = lpo_nrmp(c, W[i]);
             if (nrmW > 1e-8) {
@@ -877,6 +898,7 @@ int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double 
         for (int i = 0; i < k; ++i) {
             orthoi_v2V(c, Vp[i], V, k);
             orthoi_v2V(c, Vp[i], W, k);
+            orthoi_v2V(c, Vp[i], V0, k0);
             orthoi_Vi(c, Vp, i);
             const double nrmVpi = lpo_nrmp(c, Vp[i]);
             if (nrmVpi > 1e-8) {
@@ -927,6 +949,7 @@ int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double 
         }
         for (int i = 0; i < k; ++i) {
             orthoi_v2V(c, V[i], V, i);
+            orthoi_v2V(c, V[i], V0, k0);
             const double temp1 = lpo_nrmp(c, V[i]);
             scale_c(c, V[i], 1.0 / temp1);
         }
@@ -946,6 +969,7 @@ int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, double 
     free(V); free(alpha); free(emp); free(Pn); free(Pw); free(Pz); free(val); free(UU); free(YY); free(Vm);
     free(rho); free(f); free(g); free(gp); free(Dx); free(Dg); free(tmpp); free(tmppF); free(nlnR); free(nlnF);
     free(xp);
+    free(V0);
     return iter;
 }
 

@@ -85,6 +85,10 @@ int lpo_Spring_Pairing(lpo_ctx *c, lpo_cplx *Phi0, lpo_cplx *Phi1, int maxiter,
  * the number of iterations; *res_out receives the last nrmp(ngrad_cam(x)). */
 int lpo_hiosd_lobpcg_sieh(lpo_ctx *c, lpo_cplx *x, lpo_cplx *V, int k, double dt, double betat, double betau,
                           double epsf, int maxiter, double *res_out);
+/* GOSD.cpp:123-531 (hiosd_lobpcg_sieh_V0): the same, with the sweep kept
+ * orthogonal to the k0 fixed fields V0 (orthonormal, read only). */
+int lpo_hiosd_lobpcg_sieh_V0(lpo_ctx *c, lpo_cplx *x, lpo_cplx *V, int k, const lpo_cplx *V0, int k0, double dt,
+                             double betat, double betau, double epsf, int maxiter, double *res_out);
 
 /* Symmetric eigensolver for the small Rayleigh-Ritz problem (cyclic Jacobi).
  * A is n*n row-major, overwritten; w receives eigenvalues ascending, Z the

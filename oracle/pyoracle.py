@@ -61,6 +61,7 @@ def lib():
             "lpo_MinMax_Forces": (None, [_c, _c, _c]),
             "lpo_Spring_Pairing": (_i, [_c, _c, _c, _i, _d, _c]),
             "lpo_hiosd_lobpcg_sieh": (_i, [_c, _c, _c, _i, _d, _d, _d, _d, _i, _c]),
+            "lpo_hiosd_lobpcg_sieh_V0": (_i, [_c, _c, _c, _i, _c, _i, _d, _d, _d, _d, _i, _c]),
             "lpo_sym_eig": (None, [_i, _c, _c, _c]),
         }
         for name, (res, args) in sig.items():
@@ -235,4 +236,14 @@ class Oracle:
         r = np.zeros(1)
         it = lib().lpo_hiosd_lobpcg_sieh(self.h, _p(x), _p(V), V.shape[0], float(dt), float(betat), float(betau),
                                          float(epsf), int(maxiter), _p(r))
+        return it, float(r[0])
+
+    def hiosd_lobpcg_sieh_V0(self, x, V, V0, dt=0.8, betat=8.0, betau=1.3, epsf=1e-9, maxiter=40000):
+        """GOSD.cpp:123-531, in place on x and V (k, N, N, N) with the fixed
+        orthonormal fields V0 (k0, N, N, N); returns (iters, last residual)."""
+        for a in (x, V, V0):
+            assert a.flags["C_CONTIGUOUS"] and a.dtype == np.complex128
+        r = np.zeros(1)
+        it = lib().lpo_hiosd_lobpcg_sieh_V0(self.h, _p(x), _p(V), V.shape[0], _p(V0), V0.shape[0], float(dt),
+                                            float(betat), float(betau), float(epsf), int(maxiter), _p(r))
         return it, float(r[0])

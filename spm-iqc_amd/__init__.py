@@ -74,6 +74,7 @@ SIGNATURES = {
     "spm_lobpcg_init": (_i, [_c, _c, _c, _i]),
     "spm_lobpcg": (_i, [_c, _c, _c, _i, _i, _c]),
     "spm_hisd": (_i, [_c, _c, _c, _i, _d, _d, _d, _d, _i, _c, _c]),
+    "spm_gosd": (_i, [_c, _c, _c, _i, _c, _i, _d, _d, _d, _d, _i, _c, _c]),
     "spm_rayleighq": (_i, [_c, _c, _c, _pd]),
     "spm_gram": (_i, [_c, _c, _i, _c, _i, _c]),
     "spm_get_tv": (_i, [_c, _c, _c, _c]),
@@ -358,6 +359,18 @@ class Grid:
         res = np.zeros(1)
         _check(lib().spm_hisd(self.h, self._real(x), self._block(V), k, float(dt), float(betat), float(betau),
                               float(epsf), int(maxiter), it.ctypes.data_as(_c), res.ctypes.data_as(_c)), "spm_hisd")
+        return int(it[0]), float(res[0])
+
+    def gosd(self, x, V, V0, dt=0.8, betat=8.0, betau=1.3, epsf=1e-9, maxiter=40000):
+        """hiosd_lobpcg_sieh_V0 (GOSD.cpp:123-531) on x and V (k, N, N, N) in place,
+        orthogonal to the fixed fields V0 (k0, N, N, N); returns (steps, last residual)."""
+        self._stream()
+        k, k0 = int(V.shape[0]), int(V0.shape[0])
+        it = np.zeros(1, dtype=np.int32)
+        res = np.zeros(1)
+        _check(lib().spm_gosd(self.h, self._real(x), self._block(V), k, self._block(V0) if k0 else None, k0,
+                              float(dt), float(betat), float(betau), float(epsf), int(maxiter),
+                              it.ctypes.data_as(_c), res.ctypes.data_as(_c)), "spm_gosd")
         return int(it[0]), float(res[0])
 
     def gram(self, X, Y):

@@ -27,6 +27,9 @@ int block_means(spm_ctx *h, const void *const *X, int m, double *out);
 int field_copy(spm_ctx *h, void *dst, const void *src);
 int field_zero(spm_ctx *h, void *dst);
 
+// Q <- Q - B (B^T Q), two rounds (B orthonormal: Q ends orthogonal to B)
+int block_project(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb);
+
 // Orthonormalise the fields Q[0..nq) in place, in order, as the reference's
 // modified Gram-Schmidt does (Orthoi_v2V / Orthoi_Vi + nrmp + scaling,
 // BasicOperators.cpp:229-255): first against the orthonormal fields

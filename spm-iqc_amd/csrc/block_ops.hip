@@ -479,13 +479,13 @@ static void chol_inv(int n, const double *S, double tol, bool deflate, bool *kee
     }
 }
 
-int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb, double tol, bool *keep)
+int block_project(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb)
 {
-    if (nq < 1) return SPM_OK;
-    if (nq > 16 || nb > 16) return spm_fail(SPM_EINVAL, "block_ortho: at most 16 + 16 fields");
+    if (nq < 1 || nb < 1) return SPM_OK;
+    if (nq > 16 || nb > 16) return spm_fail(SPM_EINVAL, "block_project: at most 16 + 16 fields");
     std::vector<const void *> qin(Q, Q + nq);
     // two rounds of Q <- Q - B (B^T Q)
-    if (nb > 0) {
+    {
         for (int round = 0; round < 2; ++round) {
             std::vector<double> P(nb * nq);
             RCHK(block_gram(h, B, nb, qin.data(), nq, P.data()));
@@ -503,6 +503,15 @@ int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb
             RCHK(block_combine(h, in.data(), nq + nb, Q, nq, coef.data(), nullptr));
         }
     }
+    return SPM_OK;
+}
+
+int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb, double tol, bool *keep)
+{
+    if (nq < 1) return SPM_OK;
+    if (nq > 16 || nb > 16) return spm_fail(SPM_EINVAL, "block_ortho: at most 16 + 16 fields");
+    std::vector<const void *> qin(Q, Q + nq);
+    RCHK(block_project(h, Q, nq, B, nb));
     // CholQR with the reference's deflation, then a second CholQR on the kept fields
     std::vector<double> S(nq * nq), Rinv(nq * nq);
     RCHK(block_gram(h, qin.data(), nq, qin.data(), nq, S.data()));

@@ -119,6 +119,7 @@ int lobpcg_init(spm_ctx *h, const void *start, void *V, int k)
 // spare block T; keep flags of V, W, Vp.
 struct LobBlock {
     std::vector<void *> V, U, W, Y, Vp, Up, T;
+    std::vector<const void *> V0; // fixed orthonormal fields the sweep stays orthogonal to (GOSD)
     std::vector<char> keepbuf;
     bool *keepV() { return (bool *)keepbuf.data(); }
     bool *keepW() { return (bool *)keepbuf.data() + V.size(); }
@@ -178,18 +179,28 @@ static int lobpcg_sweep(spm_ctx *h, const void *phi, LobBlock &b, double *ritz)
     }
     for (int i = 0; i < k; ++i) c.g->precond(c, W[i]);
     RCHK(spm_launch_status("lobpcg precond"));
-    RCHK(block_ortho(h, W.data(), k, cv(V).data(), k, 1e-8, keepW));
+    const int k0 = (int)b.V0.size();
+    if (k0 == 0) {
+        RCHK(block_ortho(h, W.data(), k, cv(V).data(), k, 1e-8, keepW));
+    } else { // against V, then V0 (GOSD.cpp:205-207): V need not be orthogonal to V0 yet
+        RCHK(block_project(h, W.data(), k, cv(V).data(), k));
+        RCHK(block_ortho(h, W.data(), k, b.V0.data(), k0, 1e-8, keepW));
+    }
     for (int i = 0; i < k; ++i) {
         if (keepW[i]) c.g->hv(c, phi, W[i], Y[i]);
         else RCHK(field_zero(h, Y[i]));
     }
-    {
+    if (k0 == 0) {
         std::vector<const void *> B(2 * k);
         for (int i = 0; i < k; ++i) {
             B[i] = V[i];
             B[k + i] = W[i];
         }
         RCHK(block_ortho(h, Vp.data(), k, B.data(), 2 * k, 1e-8, keepP));
+    } else { // against V, W, then V0 in turn (GOSD.cpp:224-228)
+        RCHK(block_project(h, Vp.data(), k, cv(V).data(), k));
+        RCHK(block_project(h, Vp.data(), k, cv(W).data(), k));
+        RCHK(block_ortho(h, Vp.data(), k, b.V0.data(), k0, 1e-8, keepP));
     }
     for (int i = 0; i < k; ++i) {
         if (keepP[i]) c.g->hv(c, phi, Vp[i], Up[i]);
@@ -238,7 +249,9 @@ static int lobpcg_sweep(spm_ctx *h, const void *phi, LobBlock &b, double *ritz)
         shift[i] = -mean[i];
     }
     RCHK(block_combine(h, cv(V).data(), k, V.data(), k, eye.data(), shift.data()));
-    RCHK(block_ortho(h, V.data(), k, nullptr, 0, -1.0, keepV));
+    // in order, each against the earlier ones and V0 (GOSD.cpp:344-350); the earlier
+    // ones are orthogonal to V0, so this is V0's projection, then CholQR
+    RCHK(block_ortho(h, V.data(), k, k0 ? b.V0.data() : nullptr, k0, -1.0, keepV));
     if (ritz)
         for (int i = 0; i < k; ++i) ritz[i] = vals[i];
     return SPM_OK;
@@ -274,12 +287,15 @@ int lobpcg_run(spm_ctx *h, const void *phi, void *Vout, int k, int maxiter, doub
 //   x <- the semi-implicit step with force f - tmpp (SOpHisdStep: real part of the spectrum);
 //   then the sweep, f = ngrad_cam(x), stop when nrmp(f) < epsf or after maxiter steps.
 // The first Dx is f - 2 tmpp + dt (the reference adds the constant dt, :185).
-int hisd_run(spm_ctx *h, void *x, void *Vout, int k, double dt, double betat, double betau, double epsf,
-             int maxiter, int *iters, double *res_out)
+// With k0 > 0 fixed fields V0 this is hiosd_lobpcg_sieh_V0 (GOSD.cpp:123-531):
+// the sweep keeps W, Vp and V orthogonal to V0 as well.
+int hisd_run(spm_ctx *h, void *x, void *Vout, int k, const void *V0, int k0, double dt, double betat, double betau,
+             double epsf, int maxiter, int *iters, double *res_out)
 {
     Ctx &c = h->c;
     LobBlock b;
     RCHK(lob_block(h, Vout, k, 7, b));
+    for (int i = 0; i < k0; ++i) b.V0.push_back(fld((void *)V0, i, c));
     void *f = fld(h->ws, 6 * k + 0, c), *g = fld(h->ws, 6 * k + 1, c), *gp = fld(h->ws, 6 * k + 2, c);
     void *Dx = fld(h->ws, 6 * k + 3, c), *Dg = fld(h->ws, 6 * k + 4, c), *tmpp = fld(h->ws, 6 * k + 5, c);
     void *xp = fld(h->ws, 6 * k + 6, c);
@@ -366,7 +382,28 @@ int spm_hisd(spm_ctx *h, void *x, void *V, int k, double dt, double betat, doubl
     if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_hisd: maxiter >= 0");
     if (!(dt > 0) || !(betau > 0) || !(betat >= betau))
         return spm_fail(SPM_EINVAL, "spm_hisd: need dt > 0 and 0 < betau <= betat");
-    return hisd_run(h, x, V, k, dt, betat, betau, epsf, maxiter, iters, res);
+    return hisd_run(h, x, V, k, nullptr, 0, dt, betat, betau, epsf, maxiter, iters, res);
+}
+
+int spm_gosd(spm_ctx *h, void *x, void *V, int k, const void *V0, int k0, double dt, double betat, double betau,
+             double epsf, int maxiter, int *iters, double *res)
+{
+    CHECK_CTX(h);
+    if (!x || !V || x == V) return spm_fail(SPM_EINVAL, "spm_gosd: bad fields");
+    if (k < 1 || k > 8) return spm_fail(SPM_EINVAL, "spm_gosd: index k must be 1..8");
+    if (k0 < 0 || 2 * k + k0 > 16) return spm_fail(SPM_EINVAL, "spm_gosd: need k0 >= 0 and 2 k + k0 <= 16");
+    if (k0 > 0) {
+        if (!V0) return spm_fail(SPM_EINVAL, "spm_gosd: null V0");
+        // V0 is read throughout: it may not overlap x or V
+        const char *a = (const char *)V0, *ae = a + (size_t)k0 * c.R * c.esz;
+        const char *xv = (const char *)x, *xe = xv + (size_t)c.R * c.esz;
+        const char *v = (const char *)V, *ve = v + (size_t)k * c.R * c.esz;
+        if ((a < xe && xv < ae) || (a < ve && v < ae)) return spm_fail(SPM_EINVAL, "spm_gosd: V0 overlaps x or V");
+    }
+    if (maxiter < 0) return spm_fail(SPM_EINVAL, "spm_gosd: maxiter >= 0");
+    if (!(dt > 0) || !(betau > 0) || !(betat >= betau))
+        return spm_fail(SPM_EINVAL, "spm_gosd: need dt > 0 and 0 < betau <= betat");
+    return hisd_run(h, x, V, k, k0 ? V0 : nullptr, k0, dt, betat, betau, epsf, maxiter, iters, res);
 }
 
 int spm_gram(spm_ctx *h, const void *X, int ma, const void *Y, int mb, double *out)

@@ -235,3 +235,60 @@ def test_hisd_matches_oracle(spm, po, seeds, torch, N, k):
     B = Vo.real.reshape(k, -1).T / N ** 1.5
     s = np.linalg.svd(A.T @ B, compute_uv=False)
     assert s.min() > 1 - 1e-8
+
+
+def _orthonormal_mean_free(N, m, seed):
+    """m seeded fields, mean-free and orthonormal under inpfp (mean of products)."""
+    F = np.random.default_rng(seed).standard_normal((m, N ** 3))
+    F -= F.mean(axis=1, keepdims=True)
+    Q, _ = np.linalg.qr(F.T)
+    return (Q.T * np.sqrt(N ** 3)).reshape(m, N, N, N).copy()
+
+
+@pytest.mark.parametrize("N,k,k0,orth", [(16, 1, 2, True), (16, 2, 3, False), (64, 1, 3, False)])
+def test_gosd_matches_oracle(spm, po, seeds, torch, N, k, k0, orth):
+    """hiosd_lobpcg_sieh_V0 (GOSD.cpp:123-531) from the same field, block and null
+    space V0, five steps: the field, the residual, the span of V, and V orthogonal
+    to V0.  orth = False starts V off V0's complement, as critical_point_v does
+    (GOSD.cpp:60-120), which the sweep's projections in the reference's order handle."""
+    print("[case] gosd vs oracle N=%d k=%d k0=%d" % (N, k, k0), flush=True)
+    steps = 5
+    x0 = _relaxed(po, seeds, N, 30, 12) + 0.01 * seeds.gaussian(N, 1.0, 13)
+    V0 = _orthonormal_mean_free(N, k0, 15)
+    o = po.Oracle(N)
+    start = _start(k, N, 14)
+    if orth:  # start in the complement of V0, as critical_point does
+        flat = start.reshape(k, -1)
+        flat -= (flat @ V0.reshape(k0, -1).T / N ** 3) @ V0.reshape(k0, -1)
+    Vo = o.lobpcg_initialization(start)
+    Vstart = Vo.real.copy()
+    xo = (x0 + 0j).copy()
+    it_o, res_o = o.hiosd_lobpcg_sieh_V0(xo, Vo, (V0 + 0j).copy(), dt=0.8, betat=8.0, betau=1.3, maxiter=steps,
+                                         epsf=-1.0)
+    with spm.Grid(N, 0) as g:
+        x = torch.as_tensor(x0).cuda()
+        V = torch.as_tensor(Vstart).cuda()
+        V0d = torch.as_tensor(V0).cuda()
+        it, res = g.gosd(x, V, V0d, dt=0.8, betat=8.0, betau=1.3, maxiter=steps, epsf=-1.0)
+        x, V = _np(x), _np(V)
+    assert it == it_o == steps
+    assert abs(res - res_o) <= 1e-8 * res_o
+    assert np.abs(x - xo.real).max() < 1e-10
+    A = V.reshape(k, -1).T / N ** 1.5
+    B = Vo.real.reshape(k, -1).T / N ** 1.5
+    s = np.linalg.svd(A.T @ B, compute_uv=False)
+    assert s.min() > 1 - 1e-8
+    assert np.abs(V.reshape(k, -1) @ V0.reshape(k0, -1).T / N ** 3).max() < 1e-10
+
+
+def test_gosd_rejects_bad_arguments(spm, torch):
+    """2 k + k0 <= 16, a V0 for k0 > 0, and no overlap of V0 with V."""
+    N = 16
+    with spm.Grid(N, 0) as g:
+        x = torch.zeros((N, N, N), dtype=torch.float64, device="cuda")
+        V = torch.zeros((2, N, N, N), dtype=torch.float64, device="cuda")
+        V0 = torch.zeros((13, N, N, N), dtype=torch.float64, device="cuda")
+        with pytest.raises(RuntimeError, match="2 k \\+ k0"):
+            g.gosd(x, V, V0, maxiter=1)
+        with pytest.raises(RuntimeError, match="overlaps"):
+            g.gosd(x, V, V, maxiter=1)

@@ -325,3 +325,48 @@ def test_hisd_eigen_sweep_is_lobpcg(po, seeds):
     W = V0.copy()
     o.lobpcg_inip(x, W, maxiter=1)
     assert np.abs(V - W).max() < 1e-12
+
+
+def test_gosd_without_null_space_is_hisd(po, seeds):
+    """hiosd_lobpcg_sieh_V0 (GOSD.cpp:123-531) with k0 = 0 is hiosd_lobpcg_sieh."""
+    N = 12
+    o = po.Oracle(N)
+    x = (seeds.gaussian(N, 0.05, 11) + 0j).copy()
+    V = (seeds.gaussian(N, 1.0, 12) + 0j).reshape((1, N, N, N)).copy()
+    x2, V2 = x.copy(), V.copy()
+    r1 = o.hiosd_lobpcg_sieh(x, V, maxiter=3, epsf=-1.0)
+    r2 = o.hiosd_lobpcg_sieh_V0(x2, V2, np.zeros((0, N, N, N), complex), dt=0.8, betat=8.0, betau=0.02,
+                                 maxiter=3, epsf=-1.0)
+    assert r1 == r2
+    assert np.array_equal(x, x2) and np.array_equal(V, V2)
+
+
+@pytest.mark.parametrize("k0", [1, 2])
+def test_gosd_deflates_the_null_space_dense(po, seeds, k0):
+    """With a tiny dt the x-step keeps an even mean-free x (to 1e-12), so the GOSD sweeps
+    run at a fixed Hessian: with V0 the k0 lowest eigenvectors of the dense
+    mean-free Hessian (N = 6), V converges to eigenvector k0 + 1, and stays
+    orthogonal to V0 (GOSD.cpp:206, 227, 347)."""
+    N = 6
+    R = N ** 3
+    y = seeds.gaussian(N, 0.5, 18)  # eigenvalues 1 - 4 about 0.02 apart
+    phi = 0.5 * (y + _flip(y))
+    phi -= phi.mean()  # the x-step sets the mean to zero (HISD.cpp:260-261)
+    o = po.Oracle(N)
+    _, G, _ = o.tables()
+    E = np.eye(R).reshape(R, N, N, N)
+    conv = np.fft.ifftn(G[None] * np.fft.fftn(E, axes=(1, 2, 3)), axes=(1, 2, 3)).real.reshape(R, R)
+    A = conv + np.diag((-TAU - 2 * GAMMA * phi + 3 * phi * phi).ravel())
+    Q, _ = np.linalg.qr(np.eye(R)[:, 1:] - 1.0 / R)
+    ev, Z = np.linalg.eigh(Q.T @ (0.5 * (A + A.T)) @ Q)
+    assert ev[k0] - ev[k0 - 1] > 1e-3 and ev[k0 + 1] - ev[k0] > 1e-3  # separated
+    vecs = (Q @ Z[:, : k0 + 1]).T * np.sqrt(R)  # unit in the inpfp norm (mean of squares)
+    V0 = (vecs[:k0].reshape(k0, N, N, N) + 0j).copy()
+    x = (phi + 0j).copy()
+    V = (seeds.gaussian(N, 1.0, 14) + 0j).reshape((1, N, N, N)).copy()
+    o.hiosd_lobpcg_sieh_V0(x, V, V0, dt=1e-16, betat=8.0, betau=0.02, maxiter=200, epsf=-1.0)
+    assert np.abs(x.real - phi).max() < 1e-12
+    v = V[0].real
+    assert np.abs(np.einsum("aijk,ijk->a", V0.real, v) / R).max() < 1e-12
+    assert abs(o.rayleighq(x, V[0]) - ev[k0]) < 1e-10
+    assert abs(abs(np.mean(v * vecs[k0].reshape(N, N, N))) - 1.0) < 1e-9
