@@ -96,6 +96,10 @@ void copy_option_V(Option *option1, Option *option2, int k);
 void critical_point_HISD();
 void critical_point_HISD_v();
 void hiosd_lobpcg_sieh(fftw_complex *x, Option *option);
+/* GOSD.cpp:13-531 (include/GOSD.h); V0: k0 device fields */
+void critical_point();
+void critical_point_v();
+void hiosd_lobpcg_sieh_V0(fftw_complex *x, Option *option, fftw_complex **V0, int k0);
 void copy_option_V(fftw_complex *v, Option *option2, int k);
 void copy_option_V(fftw_complex **V, Option *option2, int k);
 

@@ -488,6 +488,80 @@ void critical_point_HISD_v()
     hiosd_lobpcg_sieh(u_R_local, option2);
 }
 
+// GOSD.cpp:123-531 through spm_gosd: hiosd_lobpcg_sieh with the sweep kept
+// orthogonal to the k0 device fields V0 (orthonormal); x is left in u_R_local
+// and its spectrum in u_F_local, as in hiosd_lobpcg_sieh.
+void hiosd_lobpcg_sieh_V0(fftw_complex *x, Option *option, fftw_complex **V0, int k0)
+{
+    const int k = option->k;
+    to_real(x, g_re[0]);
+    double *blk = block(k);
+    for (int i = 0; i < k; i++) to_real(option->V[i], blk + i * R());
+    double *v0 = nullptr;
+    if (k0 > 0) {
+        hck(hipMalloc(&v0, sizeof(double) * k0 * R()), "hipMalloc V0");
+        for (int i = 0; i < k0; i++) to_real(V0[i], v0 + i * R());
+    }
+    int it = 0;
+    double res = 0.0;
+    const int rc = spm_gosd(ctx(), g_re[0], blk, k, v0, k0, option->dt, option->betat, option->betau, option->epsf,
+                            option->maxiter, &it, &res);
+    if (v0) (void)hipFree(v0); // spm_gosd has synchronised its stream
+    CK(rc);
+    to_cplx(g_re[0], x);
+    for (int i = 0; i < k; i++) to_cplx(blk + i * R(), option->V[i]);
+    if (x != u_R_local) Cpy(x, u_R_local, CplxDofs_local);
+    my_fft(u_R_local, u_F_local);
+    if (myid == 0) std::printf("\n %d \t res = %.3e \n", it, res);
+}
+
+// the real part of src into dst (the drivers' [i][0] = ..., [i][1] = 0 loops)
+static void real_part_into(fftw_complex *dst, const fftw_complex *src)
+{
+    to_real(src, g_re[1]);
+    to_cplx(g_re[1], dst);
+}
+
+// GOSD.cpp:13-58 and 60-120: V0 = the first k0 = 3 modes of option1, and the
+// saddle search from u_R_local - 0.1 V_3 along V_3 (critical_point) or from
+// u_R_local + 0.1 v along v, v the HEX seed in real space (critical_point_v).
+static void gosd_driver(bool from_hex)
+{
+    const int k0 = 3;
+    hiosd_lobpcg_initialization(option2, 1);
+    option2->dt = 0.8;
+    option2->betat = 8.0;
+    option2->betau = from_hex ? 0.4 : 1.3;
+    option2->epsf = from_hex ? 1e-7 : 1e-9;
+    option2->maxiter = from_hex ? 20000 : 40000;
+    option2->outputp = 100;
+    option2->outputd = 1000;
+    if (from_hex) {
+        std::vector<fftw_complex> v((size_t)CplxDofs);
+        get_initial_value(v.data(), 3, 0.0);
+        fftw_complex *vF = dev_field(), *vR = dev_field();
+        Distribute_C(v.data(), vF);
+        my_ifft(vF, vR); // the driver's Cpy, fftw_execute(plan2), Cpy (My_FFT.cpp:16-21)
+        FuncsLinear2Cplx(u_R_local, CplxDofs_local, 1.0, u_R_local, 0.1, vR);
+        real_part_into(option2->V[0], vR);
+        (void)hipFree(vF);
+        (void)hipFree(vR);
+    } else {
+        FuncsLinear2Cplx(u_R_local, CplxDofs_local, 1.0, u_R_local, -0.1, option1->V[k0]);
+        real_part_into(option2->V[0], option1->V[k0]);
+    }
+    fftw_complex *V0[k0];
+    for (int i = 0; i < k0; i++) {
+        V0[i] = dev_field();
+        real_part_into(V0[i], option1->V[i]);
+    }
+    hiosd_lobpcg_sieh_V0(u_R_local, option2, V0, k0);
+    for (int i = 0; i < k0; i++) (void)hipFree(V0[i]);
+}
+
+void critical_point() { gosd_driver(false); }
+void critical_point_v() { gosd_driver(true); }
+
 void copy_option_V(Option *o1, Option *o2, int k)
 {
     for (int i = 0; i < k; i++) {

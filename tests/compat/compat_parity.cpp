@@ -196,12 +196,73 @@ int main(int argc, char **argv)
         e_hres = std::fabs(nrmp(fftw_Ctmp_local) - res) / res;
         Memfree_Option(h);
     }
+    double e_gx, e_gv, e_gperp;
+    {   // hiosd_lobpcg_sieh_V0, 3 steps with a null space of 2 orthonormal mean-free fields
+        const int k0 = 2;
+        std::vector<field> w(k0, field(R));
+        unsigned t = 777u;
+        for (int a = 0; a < k0; ++a) {
+            double m = 0;
+            for (size_t l = 0; l < R; ++l) {
+                t = t * 1664525u + 1013904223u;
+                w[a][l].re = (t >> 8) / 16777216.0 - 0.5;
+                m += w[a][l].re;
+            }
+            for (size_t l = 0; l < R; ++l) w[a][l].re -= m / R;
+            for (int b = 0; b < a; ++b) {  // Gram-Schmidt under the mean of products
+                double d = 0;
+                for (size_t l = 0; l < R; ++l) d += w[a][l].re * w[b][l].re;
+                for (size_t l = 0; l < R; ++l) w[a][l].re -= d / R * w[b][l].re;
+            }
+            double nn = 0;
+            for (size_t l = 0; l < R; ++l) nn += w[a][l].re * w[a][l].re;
+            for (size_t l = 0; l < R; ++l) w[a][l].re /= std::sqrt(nn / R);
+        }
+        field vu = v, w2(2 * R);
+        double nv = 0;
+        for (size_t l = 0; l < R; ++l) nv += vu[l].re * vu[l].re;
+        for (size_t l = 0; l < R; ++l) vu[l].re /= std::sqrt(nv / R);
+        for (int a = 0; a < k0; ++a)
+            for (size_t l = 0; l < R; ++l) w2[a * R + l] = w[a][l];
+        fftw_complex *V0[k0];
+        for (int a = 0; a < k0; ++a) {
+            if (hipMalloc(&V0[a], sizeof(fftw_complex) * R) != hipSuccess) std::abort();
+            up(V0[a], w[a]);
+        }
+        Option *h = (Option *)std::malloc(sizeof(Option));
+        hiosd_lobpcg_initialization(h, 1);
+        h->dt = 0.8;
+        h->betat = 8.0;
+        h->betau = 1.3;
+        h->epsf = 1e-30;
+        h->maxiter = 3;
+        up(h->V[0], vu);
+        up(u_R_local, phi);
+        hiosd_lobpcg_sieh_V0(u_R_local, h, V0, k0);
+        field x = phi, vr = vu;
+        double res = 0;
+        lpo_hiosd_lobpcg_sieh_V0(o, FP(x), FP(vr), 1, FP(w2), k0, 0.8, 8.0, 1.3, 1e-30, 3, &res);
+        e_gx = maxdiff_re(down(u_R_local), x);
+        const field vg = down(h->V[0]);
+        double dot = 0, ng = 0, nr = 0;
+        for (size_t l = 0; l < R; ++l) dot += vg[l].re * vr[l].re, ng += vg[l].re * vg[l].re, nr += vr[l].re * vr[l].re;
+        e_gv = 1.0 - std::fabs(dot) / std::sqrt(ng * nr);
+        e_gperp = 0;
+        for (int a = 0; a < k0; ++a) {
+            double d = 0;
+            for (size_t l = 0; l < R; ++l) d += vg[l].re * w[a][l].re;
+            e_gperp = std::fmax(e_gperp, std::fabs(d / R));
+            (void)hipFree(V0[a]);
+        }
+        Memfree_Option(h);
+    }
     std::printf("{\"fft\": %.3e, \"ifft\": %.3e, \"energy_rel\": %.3e, \"ngrad\": %.3e, \"hv\": %.3e, "
                 "\"precond\": %.3e, \"inpfp\": %.3e, \"gradflow\": %.3e, \"tv\": %.3e, \"natural\": %.3e, "
                 "\"minmax\": %.3e, \"spring\": %.3e, \"file\": %.3e, \"lobpcg_lowest\": %.3e, "
-                "\"hisd_x\": %.3e, \"hisd_v\": %.3e, \"hisd_res_rel\": %.3e}\n",
+                "\"hisd_x\": %.3e, \"hisd_v\": %.3e, \"hisd_res_rel\": %.3e, \"gosd_x\": %.3e, \"gosd_v\": %.3e, "
+                "\"gosd_perp\": %.3e}\n",
                 e_fft, e_ifft, e_E, e_grad, e_hv, e_pc, e_inp, e_gf, e_tv, e_nat, e_mm, e_sp, e_file,
-                std::fabs(lam - lam_ref), e_hx, e_hv1, e_hres);
+                std::fabs(lam - lam_ref), e_hx, e_hv1, e_hres, e_gx, e_gv, e_gperp);
     lpo_destroy(o);
     Memfree_global_kspace();
     Memfree_global_data();

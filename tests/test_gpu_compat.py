@@ -41,6 +41,10 @@ def test_compat_functions_match_oracle(N):
     assert res["hisd_x"] < 1e-10
     assert res["hisd_v"] < 1e-8
     assert res["hisd_res_rel"] < 1e-8
+    # hiosd_lobpcg_sieh_V0: 3 GOSD steps with a 2-field null space
+    assert res["gosd_x"] < 1e-10
+    assert res["gosd_v"] < 1e-8
+    assert res["gosd_perp"] < 1e-10
 
 
 def test_stage1_driver(seeds, tmp_path):
