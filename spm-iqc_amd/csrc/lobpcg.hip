@@ -296,7 +296,7 @@ int hisd_run(spm_ctx *h, void *x, void *Vout, int k, const void *V0, int k0, dou
     LobBlock b;
     RCHK(lob_block(h, Vout, k, 7, b));
     for (int i = 0; i < k0; ++i) b.V0.push_back(fld((void *)V0, i, c));
-    void *f = fld(h->ws, 6 * k + 0, c), *g = fld(h->ws, 6 * k + 1, c), *gp = fld(h->ws, 6 * k + 2, c);
+    void *f = fld(h->ws, 6 * k + 0, c), *gp = fld(h->ws, 6 * k + 2, c); // field 6k + 1: unused
     void *Dx = fld(h->ws, 6 * k + 3, c), *Dg = fld(h->ws, 6 * k + 4, c), *tmpp = fld(h->ws, 6 * k + 5, c);
     void *xp = fld(h->ws, 6 * k + 6, c);
     RCHK(block_ortho(h, b.V.data(), k, nullptr, 0, -1.0, b.keepV())); // :151-156
@@ -323,9 +323,13 @@ int hisd_run(spm_ctx *h, void *x, void *Vout, int k, const void *V0, int k0, dou
     int iter = 0;
     while (iter < maxiter) {
         RCHK(pv(2.0));
-        RCHK(lin2(g, f, 1.0, tmpp, -1.0, 0.0));
-        RCHK(lin2(Dg, g, 1.0, gp, -1.0, 0.0));
-        RCHK(field_copy(h, gp, g));
+        {   // g = f - tmpp, Dg = g - gp, gp <- g in one pass (:199-204); each output sums
+            // its inputs in the order f, tmpp, gp, so Dg rounds as (f - tmpp) - gp
+            const void *in[3] = {f, tmpp, gp};
+            void *o[2] = {Dg, gp};
+            const double coef[6] = {1.0, 1.0, -1.0, -1.0, -1.0, 0.0};
+            RCHK(block_combine(h, in, 3, o, 2, coef, nullptr));
+        }
         double dd[2];
         {
             const void *X[2] = {Dx, Dg}, *Yv[1] = {Dg};
