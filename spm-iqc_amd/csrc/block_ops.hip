@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256) k_gram_small(FieldPtrs X, int ma, FieldPt
         }
     };
     const long stride = (long)gridDim.x * blockDim.x * 4;
-#pragma unroll 2
+#pragma unroll 1 // 102 VGPRs, 4 waves per SIMD (unrolled twice: 174, 2 waves, FP32 Grams slower)
     for (long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < R; r += stride) {
         double x[SMALL_M][4], y[SMALL_M][4];
 #pragma unroll
