@@ -370,3 +370,16 @@ def test_gosd_deflates_the_null_space_dense(po, seeds, k0):
     assert np.abs(np.einsum("aijk,ijk->a", V0.real, v) / R).max() < 1e-12
     assert abs(o.rayleighq(x, V[0]) - ev[k0]) < 1e-10
     assert abs(abs(np.mean(v * vecs[k0].reshape(N, N, N))) - 1.0) < 1e-9
+
+
+def test_hisd_step_clamp_keeps_a_nan_step(po):
+    """HISD.cpp:203-208 clamps bta with std::min(bta, betat dt) then std::max(bta,
+    betau dt), which return their first argument when it is NaN. At a critical
+    point (x = 0: f = 0, so Dg = 0 and bta = 0/0) the step is NaN, and so is x."""
+    N = 8
+    o = po.Oracle(N)
+    x = np.zeros((N, N, N), complex)
+    V = (np.random.default_rng(3).standard_normal((1, N, N, N)) + 0j).copy()
+    with np.errstate(all="ignore"):
+        o.hiosd_lobpcg_sieh(x, V, maxiter=1, epsf=-1.0)
+    assert np.isnan(x.real).all()
