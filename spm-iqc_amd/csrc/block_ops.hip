@@ -12,7 +12,8 @@ namespace spm {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int GRAM_BLOCKS = 1024; // blocks of 4 waves; fixed, so the summation order is fixed
+constexpr int GRAM_BLOCKS = 2048; // MFMA Gram blocks of 4 waves (8 per SIMD); fixed, so the summation order is fixed
+constexpr int GRAM_BLOCKS_SMALL = 1024; // streaming Gram: 4 waves per SIMD by its registers
 constexpr int GRAM_TILE = 32;     // the Gram is computed as a padded 32 x 32 block
 
 // Each wave accumulates C = X^T Y over its 16-point chunks with
@@ -316,7 +317,8 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     for (int a = 0; a < ma; ++a) xp.p[a] = X[a];
     for (int b = 0; b < mb; ++b) yp.p[b] = Y[b];
     const int ta = ma > 16 ? 2 : 1, tb = mb > 16 ? 2 : 1;
-    const int nblk = (int)std::min<long>(GRAM_BLOCKS, std::max<long>(1, c.R / 64));
+    const bool small = ma <= SMALL_M && mb <= SMALL_M;
+    const int nblk = (int)std::min<long>(small ? GRAM_BLOCKS_SMALL : GRAM_BLOCKS, std::max<long>(1, c.R / 64));
     double *part = gram_part(h);
 #define GRAM_LAUNCH(T, A, B, S)                                                                                \
     hipLaunchKernelGGL((k_gram<T, A, B, S>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
@@ -324,7 +326,7 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     hipLaunchKernelGGL((k_gram_small<T, SAME>), dim3(nblk), dim3(256), 0, c.stream, xp, ma, yp, mb, c.R, part)
     // R = N^3 with 4 | N: a multiple of 64, so every thread's 4-point group is whole
     const bool same = ma == mb && std::equal(X, X + ma, Y);
-    if (ma <= SMALL_M && mb <= SMALL_M) {
+    if (small) {
         if (c.prec == SPM_FP64) {
             if (same) GRAM_SMALL(double, true);
             else GRAM_SMALL(double, false);
