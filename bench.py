@@ -90,15 +90,20 @@ def time_gpu(spm, torch, N, prec, steps, warmup, phi0, shape=0, passes=0):
     it, hist = g.gf_end(phi, total + 1)
     assert it == total, (it, total)
     assert np.all(np.isfinite(hist)), "non-finite energy in the timed steps"
+    E_after = g.energy(phi)  # get_Ene of the field after the last step
+    field = phi.double().cpu().numpy()
     g.close()
-    return dict(wall_s=wall, event_ms=ev_ms, hist=hist, ctx_bytes=g.ctx_bytes)
+    return dict(wall_s=wall, event_ms=ev_ms, hist=hist, ctx_bytes=g.ctx_bytes, E_after=E_after, field=field)
 
 
-def cpu_baseline(po, N, phi0, budget_s=20.0):
+def cpu_baseline(po, N, phi0, budget_s=20.0, check_fields=None):
     """The oracle (the reference's FP64 algorithm, OpenMP threads) on a bounded
-    sample of the same workload: gradient-flow steps at the same N."""
+    sample of the same workload: gradient-flow steps at the same N.  As the
+    checker it also evaluates get_Ene (LPsrc.cpp:8-38) at the GPU runs' final
+    fields (check_fields: name -> real field); those evaluations are not timed."""
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     o = po.Oracle(N, nthreads=nthreads)
+    energies = {name: o.energy_of(f) for name, f in (check_fields or {}).items()}
     uR = phi0.astype(np.complex128)
     uF = o.my_fft(uR)
     o.get_min_u_F(uR, uF, 1, -1.0)  # warm-up (page-in of the work arrays)
@@ -110,7 +115,12 @@ def cpu_baseline(po, N, phi0, budget_s=20.0):
         n += 1
     o.close()
     return dict(value=n / t, unit="steps/s", cores=nthreads, kind="port",
-                sample="%d FP64 gradient-flow steps of the oracle at %d^3 (%.1f s)" % (n, N, t))
+                sample="%d FP64 gradient-flow steps of the oracle at %d^3 (%.1f s)" % (n, N, t)), energies
+
+
+SERVE_WORKLOAD = ("reduced serving stream: %d Poisson requests at %.0f/s, 60%% relax (N 64/128/256, to "
+                  "|grad F| 1e-6 or 2000 steps), 25%% 4-mode LOBPCG (N 64/128), 15%% SPM 100 iterations (N 128); "
+                  "the 500-request stream with relaxations to the minimum is profiles/serving_500.md")
 
 
 def serve_stream(n, rate):
@@ -123,8 +133,7 @@ def serve_stream(n, rate):
     recs, st = serve.serve(reqs, states, mixed=True)
     base, sb = serve.serve_one_at_a_time(reqs, states)
     keep = ("requests", "wall_s", "throughput_rps", "p50_s", "p99_s", "errors")
-    return {"requests": n, "rate_rps": rate, "mix": "60% relax (N 64/128/256, to |grad F| 1e-6 or 2000 steps), "
-            "25% 4-mode LOBPCG (N 64/128), 15% SPM 100 iterations (N 128)",
+    return {"requests": n, "rate_rps": rate, "workload": SERVE_WORKLOAD % (n, rate),
             "p99_s": round(st["p99_s"], 4), "p50_s": round(st["p50_s"], 4),
             "mixed_batched": {k: st[k] for k in keep},
             "one_at_a_time_fp64": {k: sb[k] for k in keep},
@@ -193,12 +202,24 @@ def main():
     if args.serve_requests > 0:
         serving = serve_stream(args.serve_requests, args.serve_rate)
 
-    cpu = None
+    cpu, E_oracle = None, {}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po
 
-        cpu = cpu_baseline(po, N, phi0, args.cpu_budget)
+        cpu, E_oracle = cpu_baseline(po, N, phi0, args.cpu_budget,
+                                     check_fields={"fp64": r64["field"], "fp32": r32["field"]})
+    # energy after the last step: the mixed run against the FP64 run, both
+    # evaluated by the oracle's get_Ene at the final fields (FP32 steps only;
+    # the mixed pipeline ends with an FP64 polish, spm-iqc_amd/pipeline.py)
+    mixed_err = {"E_fp64": r64["E_after"], "E_mixed": r32["E_after"],
+                 "E_rel_mixed_vs_fp64": abs(r32["E_after"] - r64["E_after"]) / abs(r64["E_after"])}
+    if E_oracle:
+        mixed_err.update({
+            "E_oracle_at_fp64_field": E_oracle["fp64"], "E_oracle_at_mixed_field": E_oracle["fp32"],
+            "E_rel_mixed_vs_fp64_oracle": abs(E_oracle["fp32"] - E_oracle["fp64"]) / abs(E_oracle["fp64"]),
+            "E_rel_fp64_gpu_vs_oracle": abs(r64["E_after"] - E_oracle["fp64"]) / abs(E_oracle["fp64"]),
+        })
 
     out = {
         "metric": "gradient-flow steps/s at 256^3 FP64 vs mixed, % HBM roofline; p99 solve latency",
@@ -217,11 +238,14 @@ def main():
                                "Tau=-0.0061, Gamma=0.30, dt=0.3" % N, "N": N, "batch": 1,
                    "step_passes": passes,
                    "fft_shape_fp64": shapes[tuned[spm.FP64][0]] if passes == 4 else None,
-                   "fft_shape_fp32": shapes[tuned[spm.FP32][0]] if passes == 4 else None},
+                   "fft_shape_fp32": shapes[tuned[spm.FP32][0]] if passes == 4 else None,
+                   "serving_workload": (SERVE_WORKLOAD % (args.serve_requests, args.serve_rate)
+                                        if args.serve_requests > 0 else None)},
         "mixed": {"dtype": "f32 fields/FFTs, f64 sums", "value": round(sps32, 3),
                   "ms_per_step": round(r32["event_ms"] / args.steps, 4),
                   "speedup_vs_fp64": round(sps32 / sps64, 3),
-                  "roofline_frac": round(ach32 / HBM_PEAK, 4)},
+                  "roofline_frac": round(ach32 / HBM_PEAK, 4),
+                  "energy_error": mixed_err},
         "roofline": {"bound": "hbm",
                      "kernel": ("gradient-flow step (2 launches: j stage B + i-axis FFT + update + inverse + B^-1; "
                                 "j stage A^-1 + k-axis c2r + f(phi) + r2c + A)" if passes == 2 else

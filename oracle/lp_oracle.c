@@ -34,6 +34,22 @@
 
 #define MAXFACT 40
 
+/* Planted mistakes, for the CPU tests that show each closed-form pin of this
+ * oracle fails when the term it pins is wrong (tests/test_oracle_pins.py
+ * builds copies with -DLPO_PLANT=n).  Every normal build has LPO_PLANT 0:
+ *   1 SPM force step size beta 0.5 instead of 1 (SPM.cpp:52, 121)
+ *   2 MinMax_Forces tangent coefficient 1 instead of 2 (SPM.cpp:124)
+ *   3 Natural_Forces tangent coefficient 2 instead of 1 (SPM.cpp:54-55)
+ *   4 HiSD reflection 1 <f,V>V instead of 2 <f,V>V (HISD.cpp:192-193)
+ *   5 HiSD step |<Dx,Dg>/<Dx,Dx>| instead of |<Dx,Dg>/<Dg,Dg>| (HISD.cpp:205)
+ *   6 HiSD step without its upper clamp betat*dt (HISD.cpp:207)
+ *   7 lobpcg_initialization orthonormalises the fields last to first (LOBPCG.cpp:81-93)
+ *   8 lobpcg_initialization keeps the fields' means (LOBPCG.cpp:67-79)
+ *   9 SPM force adds the tangent term instead of subtracting it (SPM.cpp:84-85) */
+#ifndef LPO_PLANT
+#define LPO_PLANT 0
+#endif
+
 typedef struct {
     int n;
     int nf;
@@ -529,7 +545,7 @@ void lpo_lobpcg_initialization(lpo_ctx *c, int k, const double *start, lpo_cplx 
         V[i] = (double *)malloc(sizeof(double) * R);
         memcpy(V[i], start + (long)i * R, sizeof(double) * R);
     }
-    for (int i = 0; i < k; ++i) {
+    for (int i = 0; i < k && LPO_PLANT != 8; ++i) {
         double temp1;
         double *Vi = V[i];
         BLOCK_SUM(c, Vi[i], temp1);
@@ -537,6 +553,12 @@ void lpo_lobpcg_initialization(lpo_ctx *c, int k, const double *start, lpo_cplx 
 #pragma omp parallel for num_threads(c->nthreads) schedule(static)
         for (long j = 0; j < R; ++j) V[i][j] = V[i][j] - temp1;
     }
+    if (LPO_PLANT == 7)
+        for (int i = 0; i < k / 2; ++i) {
+            double *t = V[i];
+            V[i] = V[k - 1 - i];
+            V[k - 1 - i] = t;
+        }
     for (int i = 0; i < k; ++i) {
         for (int j = 0; j < i; ++j) {
             const double tmp = inpfp_d(c, V[i], V[j]);
@@ -548,6 +570,12 @@ void lpo_lobpcg_initialization(lpo_ctx *c, int k, const double *start, lpo_cplx 
 #pragma omp parallel for num_threads(c->nthreads) schedule(static)
         for (long l = 0; l < R; ++l) V[i][l] = a1 * V[i][l];
     }
+    if (LPO_PLANT == 7)
+        for (int i = 0; i < k / 2; ++i) {
+            double *t = V[i];
+            V[i] = V[k - 1 - i];
+            V[k - 1 - i] = t;
+        }
     for (int i = 0; i < k; ++i) {
         for (long j = 0; j < R; ++j) {
             Vout[(long)i * R + j][0] = V[i][j];
@@ -827,7 +855,7 @@ static int hisd_core(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, const lpo_cp
     while (iter < maxiter) {
         for (int i = 0; i < 3 * k; ++i) emp[i] = 1;
         PV_F(); /* :193-195 */
-        scale_c(c, tmpp, 2.0);
+        scale_c(c, tmpp, LPO_PLANT == 4 ? 1.0 : 2.0);
         for (long j = 0; j < R; ++j) { /* g = f - tmpp; Dg = g - gp; gp = g (:199-204) */
             g[j][0] = 1.0 * f[j][0] + -1.0 * tmpp[j][0];
             g[j][1] = 1.0 * f[j][1] + -1.0 * tmpp[j][1];
@@ -836,8 +864,9 @@ static int hisd_core(lpo_ctx *c, lpo_cplx *x, lpo_cplx *Vio, int k, const lpo_cp
             gp[j][0] = 1.0 * g[j][0];
             gp[j][1] = 1.0 * g[j][1];
         }
-        double bta = fabs(lpo_inpfp(c, Dx, Dg) / lpo_inpfp(c, Dg, Dg)); /* :205-212 */
-        bta = betat * dt < bta ? betat * dt : bta; /* std::min, std::max: a NaN step stays NaN */
+        /* :205-212 */
+        double bta = fabs(lpo_inpfp(c, Dx, Dg) / (LPO_PLANT == 5 ? lpo_inpfp(c, Dx, Dx) : lpo_inpfp(c, Dg, Dg)));
+        if (LPO_PLANT != 6) bta = betat * dt < bta ? betat * dt : bta; /* std::min, std::max: a NaN step stays NaN */
         bta = bta < betau * dt ? betau * dt : bta;
         for (long j = 0; j < R; ++j) {
             xp[j][0] = 1.0 * x[j][0];
@@ -1001,14 +1030,16 @@ static void forces(lpo_ctx *c, lpo_cplx *x, const lpo_cplx *v, int minmax)
     lpo_cplx *tmppF = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
     lpo_cplx *nlnR = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
     lpo_cplx *nlnF = (lpo_cplx *)calloc(R, sizeof(lpo_cplx));
-    const double bta = 1.0;
+    const double bta = LPO_PLANT == 1 ? 0.5 : 1.0;
     lpo_ngrad_cam(c, x, f);
     const double tmp = lpo_inpfp(c, f, v); /* PV_Cplx (BasicOperators.cpp:360-371) */
     for (long j = 0; j < R; ++j) {
         tmpp[j][0] = tmpp[j][0] + tmp * v[j][0];
         tmpp[j][1] = 0.0;
     }
-    if (minmax) scale_c(c, tmpp, 2.0);
+    if (minmax) scale_c(c, tmpp, LPO_PLANT == 2 ? 1.0 : 2.0);
+    else if (LPO_PLANT == 3) scale_c(c, tmpp, 2.0);
+    if (LPO_PLANT == 9) scale_c(c, tmpp, -1.0);
 #pragma omp parallel for num_threads(c->nthreads) schedule(static)
     for (long i = 0; i < R; ++i) {
         c->u2[i][0] = x[i][0] * x[i][0];

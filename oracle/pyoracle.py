@@ -18,7 +18,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liblp_oracle.so")
-_lib = None
+_libs = {}
 
 _c = ctypes.c_void_p
 _d = ctypes.c_double
@@ -32,12 +32,21 @@ def build() -> str:
     return _LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def build_planted(plant: int, outdir: str) -> str:
+    """A copy of the oracle built with the planted mistake `plant` (see
+    LPO_PLANT in lp_oracle.c), for the tests that show a pin catches it."""
+    out = os.path.join(outdir, "liblp_oracle_plant%d.so" % plant)
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-DLPO_PLANT=%d" % plant,
+                    "-o", out, os.path.join(_HERE, "lp_oracle.c"), "-lm"], check=True)
+    return out
+
+
+def lib(path=None):
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         sig = {
             "lpo_create": (_c, [_i, _d, _d, _d, _d, _d, _i]),
             "lpo_destroy": (None, [_c]),
@@ -68,8 +77,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a: np.ndarray):
@@ -94,17 +103,18 @@ def sym_eig(A: np.ndarray):
 class Oracle:
     """One reference 'process' holding the whole N^3 grid (numprocs = 1)."""
 
-    def __init__(self, N, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, nthreads=0):
+    def __init__(self, N, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, nthreads=0, libpath=None):
         self.N = int(N)
         self.R = self.N ** 3
         self.params = dict(L=L, kscale=kscale, Tau=Tau, Gamma=Gamma, dt=dt)
-        self.h = lib().lpo_create(self.N, L, kscale, Tau, Gamma, dt, int(nthreads))
+        self._L = lib(libpath)
+        self.h = self._L.lpo_create(self.N, L, kscale, Tau, Gamma, dt, int(nthreads))
         if not self.h:
             raise ValueError("oracle: bad grid size %r" % (N,))
 
     def close(self):
         if self.h:
-            lib().lpo_destroy(self.h)
+            self._L.lpo_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -128,32 +138,32 @@ class Oracle:
 
     def tables(self):
         ksq, G, ikt2 = (np.empty((self.N,) * 3) for _ in range(3))
-        lib().lpo_tables(self.h, _p(ksq), _p(G), _p(ikt2))
+        self._L.lpo_tables(self.h, _p(ksq), _p(G), _p(ikt2))
         return ksq, G, ikt2
 
     # --- My_FFT ----------------------------------------------------------
     def my_fft(self, x):
         x = self._c(x)
         y = np.empty_like(x)
-        lib().lpo_my_fft(self.h, _p(x), _p(y))
+        self._L.lpo_my_fft(self.h, _p(x), _p(y))
         return y
 
     def my_ifft(self, x):
         x = self._c(x)
         y = np.empty_like(x)
-        lib().lpo_my_ifft(self.h, _p(x), _p(y))
+        self._L.lpo_my_ifft(self.h, _p(x), _p(y))
         return y
 
     # --- BasicOperators ------------------------------------------------
     def inpfp(self, p, q):
-        return lib().lpo_inpfp(self.h, _p(self._c(p)), _p(self._c(q)))
+        return self._L.lpo_inpfp(self.h, _p(self._c(p)), _p(self._c(q)))
 
     def nrmp(self, p):
-        return lib().lpo_nrmp(self.h, _p(self._c(p)))
+        return self._L.lpo_nrmp(self.h, _p(self._c(p)))
 
     # --- LPsrc -----------------------------------------------------------
     def get_Ene(self, u_R, u_F):
-        return lib().lpo_get_Ene(self.h, _p(self._c(u_R)), _p(self._c(u_F)))
+        return self._L.lpo_get_Ene(self.h, _p(self._c(u_R)), _p(self._c(u_F)))
 
     def energy_of(self, phi_real):
         """E of a real field: u_F = my_fft(phi), then get_Ene."""
@@ -166,37 +176,37 @@ class Oracle:
         E = np.zeros(max_iter)
         res = np.zeros(max_iter)
         g = np.zeros(max_iter) if gnorm else None
-        it = lib().lpo_get_min_u_F(self.h, _p(u_R), _p(u_F), int(max_iter), float(res_tol),
+        it = self._L.lpo_get_min_u_F(self.h, _p(u_R), _p(u_F), int(max_iter), float(res_tol),
                                    _p(E), _p(res), _p(g) if gnorm else None)
         return it, E[:it], res[:it], (g[:it] if gnorm else None)
 
     def ngrad_cam(self, phir):
         phir = self._c(phir)
         g = np.empty_like(phir)
-        lib().lpo_ngrad_cam(self.h, _p(phir), _p(g))
+        self._L.lpo_ngrad_cam(self.h, _p(phir), _p(g))
         return g
 
     # --- LOBPCG ----------------------------------------------------------
     def hv_cam(self, phir, v):
         phir, v = self._c(phir), self._c(v)
         hv = np.empty_like(v)
-        lib().lpo_hv_cam(self.h, _p(phir), _p(v), _p(hv))
+        self._L.lpo_hv_cam(self.h, _p(phir), _p(v), _p(hv))
         return hv
 
     def precond_camnew(self, w):
         w = np.array(self._c(w), copy=True)
-        lib().lpo_precond_camnew(self.h, _p(w))
+        self._L.lpo_precond_camnew(self.h, _p(w))
         return w
 
     def rayleighq(self, x, v):
-        return lib().lpo_rayleighq(self.h, _p(self._c(x)), _p(self._c(v)))
+        return self._L.lpo_rayleighq(self.h, _p(self._c(x)), _p(self._c(v)))
 
     def lobpcg_initialization(self, start):
         """start: (k, N, N, N) float64 random block; returns V (k, N, N, N) complex."""
         start = np.ascontiguousarray(start, dtype=np.float64)
         k = start.shape[0]
         V = np.zeros((k,) + (self.N,) * 3, dtype=np.complex128)
-        lib().lpo_lobpcg_initialization(self.h, k, _p(start), _p(V))
+        self._L.lpo_lobpcg_initialization(self.h, k, _p(start), _p(V))
         return V
 
     def lobpcg_inip(self, x, V, maxiter=21):
@@ -204,28 +214,28 @@ class Oracle:
         assert V.flags["C_CONTIGUOUS"] and V.dtype == np.complex128
         k = V.shape[0]
         ritz = np.zeros(k)
-        lib().lpo_lobpcg_inip(self.h, _p(self._c(x)), _p(V), k, int(maxiter), _p(ritz))
+        self._L.lpo_lobpcg_inip(self.h, _p(self._c(x)), _p(V), k, int(maxiter), _p(ritz))
         return ritz
 
     # --- SPM ---------------------------------------------------------------
     def get_Tv(self, Phi0, Phi1):
         v = np.empty_like(self._c(Phi0))
-        lib().lpo_get_Tv(self.h, _p(self._c(Phi0)), _p(self._c(Phi1)), _p(v))
+        self._L.lpo_get_Tv(self.h, _p(self._c(Phi0)), _p(self._c(Phi1)), _p(v))
         return v
 
     def Natural_Forces(self, x, v):
         """In place on x."""
         assert x.flags["C_CONTIGUOUS"] and x.dtype == np.complex128
-        lib().lpo_Natural_Forces(self.h, _p(x), _p(self._c(v)))
+        self._L.lpo_Natural_Forces(self.h, _p(x), _p(self._c(v)))
 
     def MinMax_Forces(self, x, v):
         assert x.flags["C_CONTIGUOUS"] and x.dtype == np.complex128
-        lib().lpo_MinMax_Forces(self.h, _p(x), _p(self._c(v)))
+        self._L.lpo_MinMax_Forces(self.h, _p(x), _p(self._c(v)))
 
     def Spring_Pairing(self, Phi0, Phi1, maxiter, tol):
         """In place on Phi0, Phi1; returns (iters, last residual)."""
         r = np.zeros(1)
-        it = lib().lpo_Spring_Pairing(self.h, _p(Phi0), _p(Phi1), int(maxiter), float(tol), _p(r))
+        it = self._L.lpo_Spring_Pairing(self.h, _p(Phi0), _p(Phi1), int(maxiter), float(tol), _p(r))
         return it, float(r[0])
 
     # --- HiSD --------------------------------------------------------------
@@ -234,7 +244,7 @@ class Oracle:
         assert x.flags["C_CONTIGUOUS"] and x.dtype == np.complex128
         assert V.flags["C_CONTIGUOUS"] and V.dtype == np.complex128
         r = np.zeros(1)
-        it = lib().lpo_hiosd_lobpcg_sieh(self.h, _p(x), _p(V), V.shape[0], float(dt), float(betat), float(betau),
+        it = self._L.lpo_hiosd_lobpcg_sieh(self.h, _p(x), _p(V), V.shape[0], float(dt), float(betat), float(betau),
                                          float(epsf), int(maxiter), _p(r))
         return it, float(r[0])
 
@@ -244,6 +254,6 @@ class Oracle:
         for a in (x, V, V0):
             assert a.flags["C_CONTIGUOUS"] and a.dtype == np.complex128
         r = np.zeros(1)
-        it = lib().lpo_hiosd_lobpcg_sieh_V0(self.h, _p(x), _p(V), V.shape[0], _p(V0), V0.shape[0], float(dt),
+        it = self._L.lpo_hiosd_lobpcg_sieh_V0(self.h, _p(x), _p(V), V.shape[0], _p(V0), V0.shape[0], float(dt),
                                             float(betat), float(betau), float(epsf), int(maxiter), _p(r))
         return it, float(r[0])
