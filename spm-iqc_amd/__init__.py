@@ -308,12 +308,14 @@ class Grid:
         _check(lib().spm_gf_steps(self.h, int(n)), "spm_gf_steps")
 
     def gf_end(self, phi=None, max_iter=0):
+        """Final field into phi (if given); returns (iters, hist).  max_iter must be
+        the gf_begin limit for the history to be returned (0: no history)."""
         self._stream()
         hist = np.zeros((max(int(max_iter), 1), 3))
         it = ctypes.c_int()
         _check(lib().spm_gf_end(self.h, self._real(phi) if phi is not None else None,
-                                hist.ctypes.data_as(_c), ctypes.byref(it)), "spm_gf_end")
-        return it.value, hist[: min(it.value, len(hist))]
+                                hist.ctypes.data_as(_c) if max_iter > 0 else None, ctypes.byref(it)), "spm_gf_end")
+        return it.value, hist[: min(it.value, len(hist))] if max_iter > 0 else hist[:0]
 
     # -- LOBPCG operators ----------------------------------------------------
     def hv(self, phi, v, hv=None):
