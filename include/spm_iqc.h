@@ -191,6 +191,25 @@ int spm_nrmp(spm_ctx *c, const void *p, double *out);
 /* FuncsLinear2Cplx (BasicOperators.cpp:17-26) on real fields: r = a1 F1 + a2 F2 */
 int spm_lin2(spm_ctx *c, void *r, double a1, const void *F1, double a2, const void *F2);
 
+/* --- No device allocation.  No counterpart in the reference: this entry point
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * ngrad_cam (LPsrc.cpp:119-173) as spm_ngrad computes it (the same kernels),
+ * without a context: the model, grid and precision are p's (shape >= 0: the
+ * tuned shape -1 is rejected), the stream is hip_stream (NULL = the legacy
+ * default stream).  Every work buffer of the call (twiddles, the G table, the
+ * spectrum work array, partial sums, scalars) lives in one static device array
+ * of SPM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; sizes whose
+ * work does not fit (N >= 192) are rejected with SPM_EINVAL before any device
+ * work.  All calls on one GPU share that array and keep its last contents, so
+ * calls that overlap on one GPU (on different streams or threads) race: order
+ * them.  The first call on a device, and a call whose (N, precision, L, kscale)
+ * differ from the previous call's there, copy in the tables and synchronise
+ * hip_stream.  *nrm (may be NULL) receives nrmp(grad); the call then
+ * synchronises hip_stream. */
+#define SPM_NOALLOC_SCRATCH_BYTES (24u * 1024u * 1024u)
+int spm_ngrad_noalloc(const spm_params *p, const void *phi, void *grad, double *nrm, void *hip_stream);
+
 /* --- compat helpers: real part of a full complex field <-> real field */
 int spm_cplx_to_real(spm_ctx *c, const void *z, void *x);
 int spm_real_to_cplx(spm_ctx *c, const void *x, void *z);

@@ -85,6 +85,7 @@ SIGNATURES = {
     "spm_inpfp": (_i, [_c, _c, _c, _pd]),
     "spm_nrmp": (_i, [_c, _c, _pd]),
     "spm_lin2": (_i, [_c, _c, _d, _c, _d, _c]),
+    "spm_ngrad_noalloc": (_i, [ctypes.POINTER(_Params), _c, _c, _pd, _c]),
     "spm_cplx_to_real": (_i, [_c, _c, _c]),
     "spm_real_to_cplx": (_i, [_c, _c, _c]),
     "spm_batch_create": (_i, [ctypes.POINTER(_Params), _i, _i, ctypes.POINTER(_c)]),

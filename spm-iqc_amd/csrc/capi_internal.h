@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/spm_iqc.h"
 #include "spm_ctx.h"
@@ -33,6 +34,13 @@ constexpr int GF_GRAPH_STEPS = 16;
 int spm_gf_enqueue(spm_ctx *h, long n);
 
 int spm_fail(int code, const std::string &msg);
+// host tables of a context: twiddles exp(-2 pi i e / N), e < N, in FP64 and FP32
+// (interleaved re, im), and k^2 of the table entry |n|^2 = n2 (get_ksquare_local,
+// Initialization.cpp:132-148, with k_i = kscale n_i / L)
+void spm_host_twiddles(int N, std::vector<double> &twd, std::vector<float> &twf);
+double spm_host_k2(double kscale, double L, int n2);
+// the checks spm_create makes of its parameters (no device work); who names the caller
+int spm_check_params(const spm_params *p, const char *who);
 // context of nmem batch members (1: a single field) with hist_cap iterations of history each
 int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out);
 void spm_ctx_free(spm_ctx *h);

@@ -138,6 +138,25 @@ extern "C" double spm_get_GK(double k2)
     return exp(-0.5 * k2 / s2) * (d0 + d2 * k2 + d4 * k4 + d6 * k6 + d8 * k8) + 0.08;
 }
 
+void spm_host_twiddles(int N, std::vector<double> &twd, std::vector<float> &twf)
+{
+    twd.resize(2 * N);
+    twf.resize(2 * N);
+    for (int e = 0; e < N; ++e) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * e / N;
+        twd[2 * e] = (double)cosl(a);
+        twd[2 * e + 1] = (double)(-sinl(a));
+        twf[2 * e] = (float)cosl(a);
+        twf[2 * e + 1] = (float)(-sinl(a));
+    }
+}
+
+double spm_host_k2(double kscale, double L, int n2)
+{
+    const double P = kscale / L;
+    return (double)((long double)P * (long double)P * (long double)n2);
+}
+
 int spm_dev_alloc(spm_ctx *h, void **p, size_t bytes)
 {
     if (hipMalloc(p, bytes) != hipSuccess) {
@@ -171,6 +190,21 @@ static bool supported_N(int N)
         return true;
     default: return false;
     }
+}
+
+int spm_check_params(const spm_params *p, const char *who)
+{
+    const std::string w(who);
+    if (!supported_N(p->N)) return spm_fail(SPM_ENOTSUP, w + ": unsupported N " + std::to_string(p->N));
+    if (p->precision != SPM_FP64 && p->precision != SPM_FP32)
+        return spm_fail(SPM_EINVAL, w + ": precision must be SPM_FP64 or SPM_FP32");
+    if (!(p->L > 0) || !(p->kscale > 0) || !std::isfinite(p->Tau) || !std::isfinite(p->Gamma) || !(p->dt > 0))
+        return spm_fail(SPM_EINVAL, w + ": L, kscale, dt must be > 0 and Tau, Gamma finite");
+    if (p->step_passes != 0 && p->step_passes != 2 && p->step_passes != 4)
+        return spm_fail(SPM_EINVAL, w + ": step_passes must be 0, 2 or 4");
+    if (p->step_passes == 2 && p->N != 64 && p->N != 128 && p->N != 256)
+        return spm_fail(SPM_ENOTSUP, w + ": the two-pass gradient-flow step exists for N = 64, 128 and 256 only");
+    return SPM_OK;
 }
 
 static int ensure_hist(spm_ctx *h, long n)
@@ -211,15 +245,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     if (!p || !out) return spm_fail(SPM_EINVAL, "spm_create: null argument");
     *out = nullptr;
     if (nmem < 1) return spm_fail(SPM_EINVAL, "spm_create: batch capacity must be >= 1");
-    if (!supported_N(p->N)) return spm_fail(SPM_ENOTSUP, "spm_create: unsupported N " + std::to_string(p->N));
-    if (p->precision != SPM_FP64 && p->precision != SPM_FP32)
-        return spm_fail(SPM_EINVAL, "spm_create: precision must be SPM_FP64 or SPM_FP32");
-    if (!(p->L > 0) || !(p->kscale > 0) || !std::isfinite(p->Tau) || !std::isfinite(p->Gamma) || !(p->dt > 0))
-        return spm_fail(SPM_EINVAL, "spm_create: L, kscale, dt must be > 0 and Tau, Gamma finite");
-    if (p->step_passes != 0 && p->step_passes != 2 && p->step_passes != 4)
-        return spm_fail(SPM_EINVAL, "spm_create: step_passes must be 0, 2 or 4");
-    if (p->step_passes == 2 && p->N != 64 && p->N != 128 && p->N != 256)
-        return spm_fail(SPM_ENOTSUP, "spm_create: the two-pass gradient-flow step exists for N = 64, 128 and 256 only");
+    RCHK(spm_check_params(p, "spm_create"));
     spm_ctx *h = new spm_ctx();
     Ctx &c = h->c;
     c.N = p->N;
@@ -269,15 +295,9 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     c.stream = h->own;
     // twiddles exp(-2 pi i e / N)
     {
-        std::vector<double> twd(2 * c.N);
-        std::vector<float> twf(2 * c.N);
-        for (int e = 0; e < c.N; ++e) {
-            const long double a = 2.0L * 3.14159265358979323846264338327950288L * e / c.N;
-            twd[2 * e] = (double)cosl(a);
-            twd[2 * e + 1] = (double)(-sinl(a));
-            twf[2 * e] = (float)cosl(a);
-            twf[2 * e + 1] = (float)(-sinl(a));
-        }
+        std::vector<double> twd;
+        std::vector<float> twf;
+        spm_host_twiddles(c.N, twd, twf);
         if ((rc = spm_dev_alloc(h, &c.tw, 2 * c.N * c.esz))) return bail(rc);
         if (hipMemcpy(c.tw, c.prec == SPM_FP64 ? (void *)twd.data() : (void *)twf.data(), 2 * c.N * c.esz,
                       hipMemcpyHostToDevice) != hipSuccess)
@@ -290,10 +310,8 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
         c.ntab = 3 * h2 * h2 + 1;
         std::vector<double> Gt(c.ntab), It(c.ntab), Bt(c.ntab), GDd(2 * c.ntab);
         std::vector<float> GDf(2 * c.ntab);
-        const double P = c.kscale / c.L;
         for (int n2 = 0; n2 < c.ntab; ++n2) {
-            const double k2 = (double)((long double)P * (long double)P * (long double)n2);
-            Gt[n2] = spm_get_GK(k2);
+            Gt[n2] = spm_get_GK(spm_host_k2(c.kscale, c.L, n2));
             It[n2] = 1.0 / (Gt[n2] + std::fabs(c.Tau) + 0.02);
             Bt[n2] = 1.0 / (1 + 1.0 * Gt[n2]);
             GDd[2 * n2] = Gt[n2];
