@@ -16,7 +16,7 @@ namespace spm {
 struct GridIface;
 
 enum ScalarSlot { // device scalars (ctx.scal)
-    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_FV, SC_TV2, SC_RES0, SC_RES1, SC_FV1,
+    SC_MEAN_F = 0, SC_GN2, SC_MEAN_CV, SC_E1, SC_E2, SC_TMP0, SC_TMP1, SC_FV, SC_TV2, SC_RES0, SC_RES1, SC_FV1, SC_DREL,
     SC_COUNT = 64
 };
 

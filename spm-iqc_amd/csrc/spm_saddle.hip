@@ -106,13 +106,14 @@ template <typename T> __global__ void k_div_by_sqrt(T *v, long n, const double *
 // The get_Tv that follows (SPM.cpp:218, and 185 of the next iteration) rides
 // along: d = Phi1' - Phi0', left unnormalised.  So do the real-space sums of the
 // next two force scalars: d^2; per replica nl_r d and nl_r
-// (nl = Tau x + Gamma x^2 - x^3); d.
+// (nl = Tau x + Gamma x^2 - x^3); d; and Phi0'^2 (the scale of the replicas).
+constexpr int SPRING_SUMS = 7;
 template <typename T>
 __global__ void __launch_bounds__(256) k_spring_sums(T *p0, T *p1, T *__restrict__ d, long n, double dt, double ks,
                                                     double Tau, double Gamma, double *__restrict__ part)
 {
-    __shared__ double red[6 * 16];
-    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    __shared__ double red[SPRING_SUMS * 16];
+    double s[SPRING_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const double a = (double)p0[i], b = (double)p1[i];
         const T a1 = (T)(1 * a + dt * (ks * b + -ks * a));
@@ -130,11 +131,12 @@ __global__ void __launch_bounds__(256) k_spring_sums(T *p0, T *p1, T *__restrict
         s[3] += n1 * dd;
         s[4] += n1;
         s[5] += dd;
+        s[6] += u0 * u0;
     }
-    block_reduce<6, 0u>(s, red);
+    block_reduce<SPRING_SUMS, 0u>(s, red);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) part[(long)q * gridDim.x + blockIdx.x] = s[q];
+        for (int q = 0; q < SPRING_SUMS; ++q) part[(long)q * gridDim.x + blockIdx.x] = s[q];
 }
 
 // The force scalars after a spring step: tv2 = nrmp(d)^2 and, for both
@@ -144,27 +146,33 @@ __global__ void __launch_bounds__(256) k_spring_sums(T *p0, T *p1, T *__restrict
 // that step is linear: with a = dt ks,
 //   x0' = (1 - a) x0 + a x1,  x1' = a (1 - a) x0 + (1 - a + a^2) x1,
 //   d' = x1' - x0' = (1 - a)^2 (x1 - x0).
+// These combinations cancel: their rounding error is about eps |x|^2 |G|
+// whatever |d|, so the force term's error grows like 1 / |d| as the replicas
+// close.  drel = |d|^2 / mean(x0^2) tells the host when to take the terms from a
+// G convolution of d instead (first_tangent), whose error is relative to |d|.
 __global__ void __launch_bounds__(256) k_spring_fin(const double *__restrict__ ps, int nbs,
                                                    const double *__restrict__ pq, int nbq, double R, double a,
-                                                   double *tv2, double *out0, double *out1)
+                                                   double *tv2, double *out0, double *out1, double *drel)
 {
-    __shared__ double red[9 * 16];
-    double s[9];
+    constexpr int NS = SPRING_SUMS + 3;
+    __shared__ double red[NS * 16];
+    double s[NS];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) s[q] = 0.0;
+    for (int q = 0; q < NS; ++q) s[q] = 0.0;
     for (int b = threadIdx.x; b < nbs; b += blockDim.x)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) s[q] += ps[(long)q * nbs + b];
+        for (int q = 0; q < SPRING_SUMS; ++q) s[q] += ps[(long)q * nbs + b];
     for (int b = threadIdx.x; b < nbq; b += blockDim.x)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) s[6 + q] += pq[(long)q * nbq + b];
-    block_reduce<9, 0u>(s, red);
+        for (int q = 0; q < 3; ++q) s[SPRING_SUMS + q] += pq[(long)q * nbq + b];
+    block_reduce<NS, 0u>(s, red);
     if (threadIdx.x == 0) {
-        const double Q00 = s[6], Q01 = s[7], Q11 = s[8];
+        const double Q00 = s[SPRING_SUMS], Q01 = s[SPRING_SUMS + 1], Q11 = s[SPRING_SUMS + 2];
         const double A[2][2] = {{1 - a, a}, {a * (1 - a), 1 - a + a * a}};
         const double D0 = -(1 - a) * (1 - a), D1 = (1 - a) * (1 - a);
         const double d2 = s[0] / R, mv = s[5] / R;
         *tv2 = d2;
+        *drel = d2 / (s[6] / R);
         double *out[2] = {out0, out1};
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
@@ -210,8 +218,8 @@ int spring_tangent(spm_ctx *h, void *p0, void *p1, void *d, double dt, double ks
 {
     Ctx &c = h->c;
     double *ps = c.slot(5);
-    // 2048 blocks (8 per CU) keep enough loads in flight; 6 partial sums per block
-    const int nbs = (int)std::min<long>(2048, c.part_slot / 6);
+    // 2048 blocks (8 per CU) keep enough loads in flight
+    const int nbs = (int)std::min<long>(2048, c.part_slot / SPRING_SUMS);
     if (c.prec == SPM_FP64)
         hipLaunchKernelGGL(k_spring_sums<double>, dim3(nbs), dim3(256), 0, c.stream, (double *)p0, (double *)p1,
                            (double *)d, c.R, dt, ks, c.Tau, c.Gamma, ps);
@@ -219,7 +227,7 @@ int spring_tangent(spm_ctx *h, void *p0, void *p1, void *d, double dt, double ks
         hipLaunchKernelGGL(k_spring_sums<float>, dim3(nbs), dim3(256), 0, c.stream, (float *)p0, (float *)p1,
                            (float *)d, c.R, dt, ks, c.Tau, c.Gamma, ps);
     hipLaunchKernelGGL(k_spring_fin, dim3(1), dim3(256), 0, c.stream, ps, nbs, fs.part, (int)fs.nb, (double)c.R,
-                       dt * ks, c.scal + SC_TV2, c.scal + SC_FV, c.scal + SC_FV1);
+                       dt * ks, c.scal + SC_TV2, c.scal + SC_FV, c.scal + SC_FV1, c.scal + SC_DREL);
     return spm_launch_status("spring");
 }
 
@@ -303,7 +311,10 @@ int spm_minmax_force(spm_ctx *h, void *x, const void *v)
 // (Parseval) from the spectra they already hold, and the spring kernel forms d
 // and the real-space sums.  So no transform of d is needed after the first.
 // ngrad_cam keeps only its norm (the residual), and its input transform is the
-// force round trip's output pass.  The host reads the residual once per iteration.
+// force round trip's output pass.  The host reads the residual once per iteration,
+// with |d|^2 / mean(Phi0^2): below CLOSE_FP64 (CLOSE_FP32), where the spectral
+// combinations have lost too many digits to cancellation, the next iteration's
+// tangents come from a G convolution of d itself, as the first one does.
 int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double tol, int *iters, double *res_out,
                        double *res_hist)
 {
@@ -322,19 +333,26 @@ int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double t
     f0.keep = keep;
     f1.other = keep;
     f1.part = c.slot(7);
+    // |d|^2 / mean(Phi0^2) below which the tangent terms come from d's own transform:
+    // the spectral form then keeps about 10 digits of the force scalar in FP64 and 5 in FP32
+    const double close = c.prec == SPM_FP64 ? 1e-12 : 1e-4;
+    bool direct = true; // until the first read-back gives the ratio
     if (res > tol && maxiter > 0) RCHK(first_tangent(h, Phi[0], Phi[1], d));
     while (res > tol && iter < maxiter) {
         iter = iter + 1;
         c.g->force(c, Phi[0], d, c.scal + SC_FV, 1.0, &f0);
         c.g->force(c, Phi[1], d, c.scal + SC_FV1, 1.0, &f1);
         RCHK(spring_tangent(h, Phi[0], Phi[1], d, dt, ks, f1));
+        if (direct) RCHK(first_tangent(h, Phi[0], Phi[1], d));
         for (int i = 0; i < 2; ++i) {
             c.g->force_ngrad(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 2.0, i ? &f1 : &f0);
             hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, c.stream, c.scal + SC_GN2, c.scal + SC_RES0 + i);
         }
         RCHK(spring_tangent(h, Phi[0], Phi[1], d, dt, ks, f1)); // and the next iteration's tangent
-        double g2[2];
-        RCHK(spm_read_scalars(h, c.scal + SC_RES0, 2, g2));
+        double g2[4]; // SC_RES0, SC_RES1, SC_FV1, SC_DREL
+        RCHK(spm_read_scalars(h, c.scal + SC_RES0, 4, g2));
+        direct = !(g2[3] >= close); // a NaN ratio (coinciding or zero replicas) as well
+        if (direct) RCHK(first_tangent(h, Phi[0], Phi[1], d));
         res = 0.0;
         for (int i = 0; i < 2; ++i) {
             const double resi = std::sqrt(g2[i]);

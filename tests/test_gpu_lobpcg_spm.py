@@ -210,6 +210,33 @@ def test_spring_pairing_matches_oracle(spm, po, seeds, torch, N):
     assert np.abs(b - P1.real).max() < 1e-11
 
 
+@pytest.mark.parametrize("N", [16, 64])
+def test_spring_pairing_close_replicas_matches_oracle(spm, po, seeds, torch, N):
+    """Replicas 1e-9 apart: the tangent terms <x_r, G d> come from d's own G
+    convolution (the spectral combinations of the replicas' quadratic forms would
+    cancel to eps |x|^2 |G|, a force error growing like 1 / |d|).  The pair's
+    separation d = Phi1 - Phi0 must follow the oracle's to 1e-4 of |d|.  At this
+    separation d itself carries rounding of eps |x| / |d| (about 5e-9 relative) in
+    any implementation, so the residual agrees to 1e-7, not 1e-10."""
+    iters = 12
+    print("[case] spring_pairing close replicas vs oracle N=%d" % N, flush=True)
+    p0 = _relaxed(po, seeds, N, 50, 3)
+    p1 = p0 + 1e-9 * seeds.gaussian(N, 1.0, 9)
+    o = po.Oracle(N)
+    P0, P1 = (p0 + 0j).copy(), (p1 + 0j).copy()
+    it_o, res_o = o.Spring_Pairing(P0, P1, iters, 1e-30)
+    with spm.Grid(N, 0) as g:
+        t0, t1 = torch.as_tensor(p0).cuda(), torch.as_tensor(p1).cuda()
+        it, res, hist = g.spring_pairing(t0, t1, iters, 1e-30)
+        a, b = _np(t0), _np(t1)
+    assert it == it_o == iters
+    assert abs(res - res_o) <= 1e-7 * res_o
+    assert np.abs(a - P0.real).max() < 1e-11
+    assert np.abs(b - P1.real).max() < 1e-11
+    d_ref = (P1 - P0).real
+    assert np.abs((b - a) - d_ref).max() < 1e-4 * np.abs(d_ref).max()
+
+
 @pytest.mark.parametrize("N,k", [(16, 1), (16, 2), (64, 1)])
 def test_hisd_matches_oracle(spm, po, seeds, torch, N, k):
     """hiosd_lobpcg_sieh (HISD.cpp:86-485) from the same field and block, five steps
