@@ -67,11 +67,13 @@ def _device(torch, x, dtype):
 
 
 def relax(N, phi0, precision="fp64", gn_tol=1e-8, max_iter=100000, polish=2000, chunk=5000, params=None,
-          verbose=False):
+          verbose=False, switch_tol=None):
     """Gradient flow from phi0 until ||grad F|| <= gn_tol or max_iter steps
     (the reference's cap, LPsrc.cpp:52).  Mixed: FP32 steps for
-    max_iter - polish steps (or until the tolerance), then FP64 steps until the
-    tolerance or the budget.  Returns (phi FP64 device tensor, info)."""
+    max_iter - polish steps (or until the tolerance, or until switch_tol when
+    given: the FP32 steps stop there and FP64 takes the field to gn_tol), then
+    FP64 steps until the tolerance or the budget.  Returns (phi FP64 device
+    tensor, info)."""
     torch = spm._torch()
     params = dict(DEFAULTS, **(params or {}))
     info = {"precision": precision}
@@ -80,7 +82,8 @@ def relax(N, phi0, precision="fp64", gn_tol=1e-8, max_iter=100000, polish=2000, 
     if precision == "mixed":
         with spm.Grid(N, spm.FP32, **params) as g32:
             p = _device(torch, phi0, torch.float32)
-            n32, last = _gradflow(g32, p, max(max_iter - polish, 0), gn_tol, chunk, verbose, "fp32")
+            n32, last = _gradflow(g32, p, max(max_iter - polish, 0), max(gn_tol, switch_tol or 0.0), chunk, verbose,
+                                  "fp32")
             torch.cuda.current_stream().synchronize()
         phi = p.double()
         del p
@@ -137,40 +140,44 @@ def lowest_modes(N, phi, precision="fp64", k=8, nwant=4, calls=10, maxiter=21, s
 
 
 def saddle(N, phi, v, precision="fp64", offset=0.01, maxiter=30000, tol=1e-9, spm_polish=2000, chunk=1000,
-           params=None, verbose=False):
-    """Spring_Pairing (SPM.cpp:170-263) from the replicas phi -/+ offset v
+           params=None, verbose=False, one_sided=False, switch_tol=None):
+    """Spring_Pairing (SPM.cpp:170-263) from the replicas phi -/+ offset v, or
+    (one_sided) phi and phi + offset v as main_SPM.cpp:109-119 builds them
     (maxiter, tol as main_SPM.cpp:122).  Mixed: FP32 replicas for maxiter -
-    spm_polish iterations, then FP64.  Returns (phi0, phi1 FP64, info)."""
+    spm_polish iterations (or until switch_tol when given), then FP64.
+    Returns (phi0, phi1 FP64, info)."""
     torch = spm._torch()
     params = dict(DEFAULTS, **(params or {}))
-    info = {"offset": offset}
+    info = {"offset": offset, "one_sided": one_sided}
     t0 = time.perf_counter()
-    p0 = phi - offset * v
+    p0 = phi.clone() if one_sided else phi - offset * v
     p1 = phi + offset * v
     n32, res = 0, float("nan")
 
-    def run(g, a, b, n, tag):
+    def run(g, a, b, n, tag, stop):
         done, r = 0, float("nan")
         while done < n:
             m = min(chunk, n - done)
-            it, r, _ = g.spring_pairing(a, b, m, tol)
+            it, r, _ = g.spring_pairing(a, b, m, stop)
             done += it
             _log(verbose, "  %s spm %6d  res %.3e" % (tag, done, r))
-            if it < m or r <= tol:
+            if it < m or r <= stop:
                 break
         return done, r
 
     if precision == "mixed":
         with spm.Grid(N, spm.FP32, **params) as g32:
             a, b = p0.float(), p1.float()
-            n32, res = run(g32, a, b, max(maxiter - spm_polish, 0), "fp32")
+            n32, res = run(g32, a, b, max(maxiter - spm_polish, 0), "fp32", max(tol, switch_tol or 0.0))
+            if switch_tol:
+                res = float("nan")  # FP64 takes the pair on to tol
             p0, p1 = a.double(), b.double()
             del a, b
     t1 = time.perf_counter()
     n64 = 0
     with spm.Grid(N, spm.FP64, **params) as g:
         if not (res <= tol):
-            n64, res = run(g, p0, p1, maxiter - n32, "fp64")
+            n64, res = run(g, p0, p1, maxiter - n32, "fp64", tol)
         info["E"] = g.energy(p1)
         info["gn"] = g.ngrad(p1)[1]
         info["E_phi0"] = g.energy(p0)
@@ -191,6 +198,53 @@ def stage1(N, phi0, precision="fp64", params=None, relax_kw=None, modes_kw=None,
     rep = {"N": N, "precision": precision, "relax": ri, "eigenvalues": [float(x) for x in ritz], "modes": mi,
            "saddle": si}
     return rep, {"phi": phi, "v1": v1, "phi0": p0, "phi1": p1}
+
+
+# One cell of the {110} shell in the reference's metric: the 12 wave vectors of
+# |n|^2 = 2 sit at |k| = 1, the minimum of G.  The disordered minimum phi = 0 has
+# an index-1 saddle above it that Spring_Pairing reaches from main_SPM's replicas.
+CELL = dict(L=2 ** 0.5, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3)
+
+
+def cell_workflow(N, precision="fp64", switch_tol=1e-6, spm_switch_tol=2e-7, verbose=False):
+    """Both stages of the reference's workflow in the {110} cell:
+    stage 1 (main_SPM.cpp:66-122): relax phi = 0, 10 x 21 LOBPCG iterations
+    (k = 2), Spring_Pairing(30000, 1e-9) from (phi, phi + 0.001 V_1); the
+    saddle's index from the lowest 4 Hessian eigenvalues (FP64 LOBPCG);
+    stage 2 (grad.cpp:90-154): the saddle + 0.001 v along its unstable mode v
+    (signed away from phi = 0, towards the ordered state), relaxed to
+    ||grad F|| <= 1e-8, and the lowest 4 eigenvalues there.
+    Mixed: the FP32 stages hand over to FP64 at switch_tol (relaxations) and
+    spm_switch_tol (Spring_Pairing, whose loop starts from res = 1e-6,
+    SPM.cpp:180, so its switch must lie below that), and both precisions end at
+    the same converged states.  Returns (report, device fields)."""
+    torch = spm._torch()
+    rep = {"N": N, "precision": precision, "params": CELL}
+    t0 = time.perf_counter()
+    phi, ri = relax(N, np.zeros((N, N, N)), precision, gn_tol=1e-8, max_iter=1000, polish=0, params=CELL,
+                    verbose=verbose, switch_tol=switch_tol)
+    ritz0, V, mi = lowest_modes(N, phi, precision, k=2, nwant=2, calls=10, seed=0, params=CELL, verbose=verbose)
+    v1 = V[0].clone()
+    del V
+    p0, p1, si = saddle(N, phi, v1, precision, offset=0.001, one_sided=True, maxiter=30000, tol=1e-9, params=CELL,
+                        verbose=verbose, switch_tol=spm_switch_tol)
+    si["distance"] = float(((p1 - p0) ** 2).mean().sqrt().item())
+    ritz_s, Vs, ci = lowest_modes(N, p1, "fp64", k=4, nwant=4, calls=10, seed=7, params=CELL, verbose=verbose)
+    si["ritz"] = [float(x) for x in ritz_s]
+    si["index"] = int(np.sum(ritz_s < -1e-6))
+    si["t_index"] = ci["t"]
+    v = Vs[0].clone()
+    del Vs
+    if float((v * p1).sum().item()) < 0:
+        v = -v
+    xm, ri2 = relax(N, p1 + 0.001 * v, precision, gn_tol=1e-8, max_iter=100000, polish=0, params=CELL,
+                    verbose=verbose, switch_tol=switch_tol)
+    ritz_m, _, cm = lowest_modes(N, xm, "fp64", k=4, nwant=4, calls=10, seed=8, params=CELL, verbose=verbose)
+    ri2["ritz"] = [float(x) for x in ritz_m]
+    ri2["index"] = int(np.sum(ritz_m < -1e-6))
+    rep.update({"relax0": ri, "eigenvalues_at_minimum": [float(x) for x in ritz0], "modes": mi, "saddle": si,
+                "minimum": ri2, "t": time.perf_counter() - t0})
+    return rep, {"saddle": p1, "minimum": xm}
 
 
 def compare(ref, rep, fields_ref, fields):
@@ -223,8 +277,41 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--workload", choices=("nucleus", "cell"), default="nucleus",
+                    help="nucleus: config 2/3 (icosahedral nucleus, L = 26); cell: both stages of the workflow in "
+                         "the {110} cell (L = sqrt 2), to a certified index-1 saddle and the minimum below it")
+    ap.add_argument("--switch-tol", type=float, default=1e-6,
+                    help="cell workload, mixed: |grad F| at which the FP32 relaxations hand over to FP64")
+    ap.add_argument("--spm-switch-tol", type=float, default=2e-7,
+                    help="cell workload, mixed: residual at which FP32 Spring_Pairing hands over to FP64")
     a = ap.parse_args(argv)
     torch = spm._torch()
+    if a.workload == "cell":
+        runs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+        reps, flds = {}, {}
+        for p in runs:
+            _log(not a.quiet, "[%s]" % p)
+            reps[p], flds[p] = cell_workflow(a.N, p, a.switch_tol, a.spm_switch_tol, verbose=not a.quiet)
+            torch.cuda.empty_cache()
+        out = {"N": a.N, "workload": "cell", "runs": reps}
+        if len(runs) == 2:
+            f, m = reps["fp64"], reps["mixed"]
+            out["mixed_vs_fp64"] = {
+                "saddle_E_rel": abs(m["saddle"]["E"] - f["saddle"]["E"]) / abs(f["saddle"]["E"]),
+                "minimum_E_rel": abs(m["minimum"]["E"] - f["minimum"]["E"]) / abs(f["minimum"]["E"]),
+                "saddle_ritz_abs": [abs(x - y) for x, y in zip(m["saddle"]["ritz"], f["saddle"]["ritz"])],
+                "minimum_ritz_abs": [abs(x - y) for x, y in zip(m["minimum"]["ritz"], f["minimum"]["ritz"])],
+                "saddle_field_maxabs": float((flds["mixed"]["saddle"] - flds["fp64"]["saddle"]).abs().max().item()),
+                "minimum_field_maxabs": float((flds["mixed"]["minimum"] - flds["fp64"]["minimum"]).abs().max().item()),
+            }
+            out["speedup"] = {"saddle": f["saddle"]["t"] / m["saddle"]["t"],
+                              "minimum": f["minimum"]["t"] / m["minimum"]["t"], "total": f["t"] / m["t"]}
+        txt = json.dumps(out, indent=1, default=float)
+        print(txt, flush=True)
+        if a.out:
+            with open(a.out, "w") as fh:
+                fh.write(txt + "\n")
+        return out
     # icosahedral 30-vector star, |q| = 1 under k = n / L, amplitude 0.1, sphere of radius 6
     phi0 = seeds.icosahedral_nucleus(a.N, 26.0, 1.0, 0.1, a.seed)
     kw = dict(relax_kw=dict(gn_tol=a.gn_tol, max_iter=a.max_iter, polish=a.polish),
