@@ -109,16 +109,39 @@ void Natural_Forces(fftw_complex *x, fftw_complex *v);
 void MinMax_Forces(fftw_complex *x, fftw_complex *v);
 void Spring_Pairing(fftw_complex **Phi, double maxiter, double tol);
 
-/* --- BasicOperators.h (the operators used by the drivers and solvers) */
+/* --- BasicOperators.h:6-50, all but the three functions that pass Eigen's
+ * MatrixXd (get_mat, sort_eig; their work happens inside spm_lobpcg).  double*
+ * fields are device arrays of CplxDofs_local reals; the Print functions copy
+ * their field to the host to print it. */
 void FuncsLinear1Cplx(fftw_complex *rslt, int n, const double a1, const fftw_complex *F1);
 void FuncsLinear2Cplx(fftw_complex *rslt, int n, const double a1, const fftw_complex *F1, const double a2,
                       const fftw_complex *F2);
+void FuncCplxAddAConst(fftw_complex *rslt, int n, const double a);
+void FuncRealAddAConst(fftw_complex *rslt, int n, const double a);
+void Print(fftw_complex *x, int n);
+void Print(double *x, int n);
+void Print(fftw_complex *x);
+void Print(double *x);
 void Cpy(fftw_complex *a, fftw_complex *b, int n);
-void FFT_dot_constant(fftw_complex *rslt, int n, double a1);
-void setCplxZero_v(fftw_complex *x, int cplxDofs);
+void norm_fft(fftw_complex *x, int n);
+double Max(double *x, int n);
 double mean_Cplx(fftw_complex *src, int n);
+void FFT_dot_constant(fftw_complex *rslt, int n, double a1);
+void Dot_constant(double *rslt, int n, double a1);
+void setCplxZero(fftw_complex rslt);
+void setCplxZero_V(fftw_complex **W, int k, int cplxDofs);
+void setCplxZero_v(fftw_complex *x, int cplxDofs);
+void Orthoi_v2V(fftw_complex *v, fftw_complex **V, int k);
+void Orthoi_Vi(fftw_complex **V, int k);
+void get_Merge(double **UU, fftw_complex **V, fftw_complex **W, fftw_complex **Vp, int k);
+int get_nozero(int *emp, int k);
+void Printf_begin_end(fftw_complex *x);
+void PV_Cplx(fftw_complex *v, fftw_complex **V, fftw_complex *tmpp, int k);
+void PV_Cplx(fftw_complex *v, fftw_complex *V, fftw_complex *tmpp, int k);
 double inpfp(fftw_complex *p, fftw_complex *q);
+double inpfp(double *p, double *q);
 double nrmp(fftw_complex *p);
+double nrmp(double *p);
 double rayleighq(fftw_complex *x, fftw_complex *v);
 
 /* --- Memfree.h:6-14 */

@@ -98,6 +98,12 @@ int spm_c2r(spm_ctx *c, const void *S, void *x);
 /* --- LPsrc */
 /* get_Ene (LPsrc.cpp:8-38) of the real field phi (u_F = FFT(phi)/N^3). */
 int spm_energy(spm_ctx *c, const void *phi, double *E);
+/* get_Ene (LPsrc.cpp:8-38) as the reference evaluates it from its two carried
+ * states: the density term from the real field phi (u_R_local), the non-local
+ * term 1/2 sum_k G |u_F_k|^2 from u_F, a full complex spectrum of N^3 values of
+ * the context precision (the reference's u_F_local layout, normalised by
+ * 1/N^3), which need not be the transform of phi. */
+int spm_energy_uf(spm_ctx *c, const void *phi, const void *u_F, double *E);
 /* ngrad_cam (LPsrc.cpp:119-173): grad = -dF/dphi; *nrm (may be NULL)
  * receives nrmp(grad) (BasicOperators.cpp:158-164). */
 int spm_ngrad(spm_ctx *c, const void *phi, void *grad, double *nrm);

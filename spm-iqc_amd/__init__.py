@@ -65,6 +65,7 @@ SIGNATURES = {
     "spm_r2c": (_i, [_c, _c, _c]),
     "spm_c2r": (_i, [_c, _c, _c]),
     "spm_energy": (_i, [_c, _c, _pd]),
+    "spm_energy_uf": (_i, [_c, _c, _c, _pd]),
     "spm_ngrad": (_i, [_c, _c, _c, _pd]),
     "spm_gradflow": (_i, [_c, _c, _i, _d, _d, _i, _c, _pi]),
     "spm_gf_begin": (_i, [_c, _c, _i, _d, _d]),
@@ -276,6 +277,14 @@ class Grid:
         self._stream()
         E = ctypes.c_double()
         _check(lib().spm_energy(self.h, self._real(phi), ctypes.byref(E)), "spm_energy")
+        return E.value
+
+    def energy_uf(self, phi, u_F) -> float:
+        """get_Ene (LPsrc.cpp:8-38) with the non-local term from the full complex
+        spectrum u_F (N, N, N) rather than from the transform of phi."""
+        self._stream()
+        E = ctypes.c_double()
+        _check(lib().spm_energy_uf(self.h, self._real(phi), self._full(u_F), ctypes.byref(E)), "spm_energy_uf")
         return E.value
 
     def ngrad(self, phi, grad=None, want_norm=True):

@@ -78,8 +78,11 @@ def _build_compat():
     """The reference-signature C++ layer over the C ABI, and the stage-1 and stage-2 drivers."""
     src = os.path.join(HERE, "compat")
     host = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include")]
-    _run(host + ["-shared", "-o", COMPAT_LIB, os.path.join(src, "compat.cpp"), "-L", LIBDIR, "-lspm_iqc",
-                 "-Wl,-rpath,$ORIGIN"])
+    # the compat layer holds a few elementwise kernels of its own (BasicOperators helpers)
+    dev = [HIPCC, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-Wall",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(HERE, "..", "include")]
+    _run(dev + ["-shared", "-o", COMPAT_LIB, os.path.join(src, "compat.cpp"), "-L", LIBDIR, "-lspm_iqc",
+                "-Wl,-rpath,$ORIGIN"])
     _run(host + ["-o", STAGE1, os.path.join(src, "main_spm.cpp"), "-L", LIBDIR, "-lspm_iqc_compat", "-lspm_iqc",
                  "-Wl,-rpath,$ORIGIN"])
     _run(host + ["-o", STAGE2, os.path.join(src, "main_grad.cpp"), "-L", LIBDIR, "-lspm_iqc_compat", "-lspm_iqc",

@@ -9,6 +9,7 @@
 // change, because the reference reads those globals at every call.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -47,6 +48,10 @@ Option *option1, *option2;
 
 // ----------------------------------------------------------------- internals
 namespace {
+
+struct cd_host {
+    double re, im;
+};
 
 spm_ctx *g_ctx = nullptr;
 double g_key[5];                   // N, L, Tau, Gamma, dt of g_ctx
@@ -324,6 +329,13 @@ void get_initial_value(fftw_complex *v, int choice, double angle)
         if (k1 < 0) k1 = N + k1;
         if (k2 < 0) k2 = N + k2;
         if (k3 < 0) k3 = N + k3;
+        // the reference writes outside u_F when a mode does not fit the grid (N too
+        // small for L); here that is an error
+        if (k1 < 0 || k1 >= N || k2 < 0 || k2 >= N || k3 < 0 || k3 >= N) {
+            std::fprintf(stderr, "spm compat: get_initial_value: mode %d of %s lies outside the %d^3 grid\n", i,
+                         path.c_str(), N);
+            std::abort();
+        }
         v[((size_t)k1 * N + k2) * N + k3][0] = f[4 * i + 3] * 0.1;
     }
 }
@@ -345,12 +357,13 @@ void my_fft(fftw_complex *x_R_local, fftw_complex *x_F_local) { CK(spm_my_fft(ct
 void my_ifft(fftw_complex *x_F_local, fftw_complex *x_R_local) { CK(spm_my_ifft(ctx(), x_F_local, x_R_local)); }
 
 // ----------------------------------------------------------------- LPsrc
-// get_Ene (LPsrc.cpp:8-38) of the field in u_R_local (u_F_local = FFT of it).
+// get_Ene (LPsrc.cpp:8-38): the density term of u_R_local, the non-local term
+// from the carried spectrum u_F_local (LPsrc.cpp:27-35), as the reference reads it.
 double get_Ene()
 {
     double E = 0.0;
     to_real(u_R_local, g_re[0]);
-    CK(spm_energy(ctx(), g_re[0], &E));
+    CK(spm_energy_uf(ctx(), g_re[0], u_F_local, &E));
     return E;
 }
 
@@ -667,6 +680,196 @@ double rayleighq(fftw_complex *x, fftw_complex *v)
     to_real(v, g_re[1]);
     CK(spm_rayleighq(ctx(), g_re[0], g_re[1], &s));
     return s;
+}
+
+// The rest of BasicOperators.h:6-50 (everything that does not pass Eigen types).
+// Fields are device arrays: fftw_complex* of n complex values, double* of n reals.
+namespace {
+
+__global__ void k_add_const(double *z, long n, double a, int both)
+{ // FuncRealAddAConst (both = 0) / FuncCplxAddAConst (both = 1), BasicOperators.cpp:28-43
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        z[2 * i] += a;
+        if (both) z[2 * i + 1] += a;
+    }
+}
+
+__global__ void k_div(double *x, long n, double d)
+{ // norm_fft's x / CplxDofs (BasicOperators.cpp:103-109)
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = x[i] / d;
+}
+
+__global__ void k_mul(double *x, long n, double a)
+{ // Dot_constant (BasicOperators.cpp:197-203)
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = a * x[i];
+}
+
+constexpr int MAX_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_max(const double *x, long n, double *part)
+{ // Max (BasicOperators.cpp:110-119): s starts at 0 and takes any larger x
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        if (s < x[i]) s = x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w /= 2) {
+        if ((int)threadIdx.x < w && red[threadIdx.x] < red[threadIdx.x + w]) red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+void launched(const char *what) { hck(hipGetLastError(), what); }
+unsigned blocks(long n) { return (unsigned)std::max<long>(1, std::min<long>(2048, (n + 255) / 256)); }
+
+std::vector<cd_host> host_copy(const fftw_complex *x, long n)
+{
+    std::vector<cd_host> h(n);
+    hck(hipMemcpy(h.data(), x, sizeof(fftw_complex) * n, hipMemcpyDeviceToHost), "Print copy");
+    return h;
+}
+
+} // namespace
+
+void FuncCplxAddAConst(fftw_complex *rslt, int n, const double a)
+{
+    hipLaunchKernelGGL(k_add_const, dim3(blocks(n)), dim3(256), 0, nullptr, (double *)rslt, (long)n, a, 1);
+    launched("FuncCplxAddAConst");
+}
+void FuncRealAddAConst(fftw_complex *rslt, int n, const double a)
+{
+    hipLaunchKernelGGL(k_add_const, dim3(blocks(n)), dim3(256), 0, nullptr, (double *)rslt, (long)n, a, 0);
+    launched("FuncRealAddAConst");
+}
+void norm_fft(fftw_complex *x, int n)
+{
+    hipLaunchKernelGGL(k_div, dim3(blocks(2L * n)), dim3(256), 0, nullptr, (double *)x, 2L * n, 1.0 * CplxDofs);
+    launched("norm_fft");
+}
+void Dot_constant(double *rslt, int n, double a1)
+{
+    hipLaunchKernelGGL(k_mul, dim3(blocks(n)), dim3(256), 0, nullptr, rslt, (long)n, a1);
+    launched("Dot_constant");
+}
+double Max(double *x, int n)
+{
+    double *part = nullptr;
+    hck(hipMalloc(&part, sizeof(double) * MAX_BLOCKS), "hipMalloc Max");
+    const unsigned nb = std::min<unsigned>(MAX_BLOCKS, blocks(n));
+    hipLaunchKernelGGL(k_max, dim3(nb), dim3(256), 0, nullptr, x, (long)n, part);
+    launched("Max");
+    std::vector<double> h(nb);
+    hck(hipMemcpy(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost), "Max read");
+    (void)hipFree(part);
+    double s = 0.0;
+    for (double v : h)
+        if (s < v) s = v;
+    return s;
+}
+void setCplxZero(fftw_complex rslt) { hck(hipMemset(rslt, 0, sizeof(fftw_complex)), "setCplxZero"); }
+void setCplxZero_V(fftw_complex **W, int k, int cplxDofs)
+{
+    for (int i = 0; i < k; i++) setCplxZero_v(W[i], cplxDofs);
+}
+// BasicOperators.cpp:229-241: v -= <v, V_i> V_i for i = 0..k-1 in turn, imaginary part 0
+void Orthoi_v2V(fftw_complex *v, fftw_complex **V, int k)
+{
+    to_real(v, g_re[0]);
+    for (int i = 0; i < k; i++) {
+        double t = 0.0;
+        to_real(V[i], g_re[1]);
+        CK(spm_inpfp(ctx(), g_re[0], g_re[1], &t));
+        CK(spm_lin2(ctx(), g_re[0], 1.0, g_re[0], -t, g_re[1]));
+    }
+    to_cplx(g_re[0], v);
+}
+// BasicOperators.cpp:243-255: V_k against V_0..V_{k-1}
+void Orthoi_Vi(fftw_complex **V, int k) { Orthoi_v2V(V[k], V, k); }
+// BasicOperators.cpp:344-358: tmpp = sum_i <v, V_i> V_i
+void PV_Cplx(fftw_complex *v, fftw_complex **V, fftw_complex *tmpp, int k)
+{
+    to_real(v, g_re[0]);
+    CK(spm_lin2(ctx(), g_re[2], 0.0, g_re[0], 0.0, g_re[0])); // tmpp = 0 (setCplxZero_v)
+    for (int i = 0; i < k; i++) {
+        double t = 0.0;
+        to_real(V[i], g_re[1]);
+        CK(spm_inpfp(ctx(), g_re[0], g_re[1], &t));
+        CK(spm_lin2(ctx(), g_re[2], 1.0, g_re[2], t, g_re[1]));
+    }
+    to_cplx(g_re[2], tmpp);
+}
+// BasicOperators.cpp:361-371: tmpp = <v, V> V
+void PV_Cplx(fftw_complex *v, fftw_complex *V, fftw_complex *tmpp, int k)
+{
+    (void)k;
+    PV_Cplx(v, &V, tmpp, 1);
+}
+// BasicOperators.cpp:145-172 on device arrays of CplxDofs_local reals
+double inpfp(double *p, double *qv)
+{
+    double s = 0.0;
+    CK(spm_inpfp(ctx(), p, qv, &s));
+    return s;
+}
+double nrmp(double *p) { return std::sqrt(inpfp(p, p)); }
+// BasicOperators.cpp:257-281: the real parts of V, W, Vp as rows 0..3k-1 of UU
+void get_Merge(double **UU, fftw_complex **V, fftw_complex **W, fftw_complex **Vp, int k)
+{
+    for (int i = 0; i < k; i++) to_real(V[i], UU[i]);
+    for (int i = 0; i < k; i++) to_real(W[i], UU[k + i]);
+    for (int i = 0; i < k; i++) to_real(Vp[i], UU[2 * k + i]);
+}
+// BasicOperators.cpp:283-294 (host flags)
+int get_nozero(int *emp, int k)
+{
+    int iter = 0;
+    for (int i = 0; i < 3 * k; i++)
+        if (emp[i] == 1) iter = iter + 1;
+    return iter;
+}
+// Print / Printf_begin_end (BasicOperators.cpp:45-92, 337-341): debug output of device fields
+void Print(fftw_complex *x, int n)
+{
+    const std::vector<cd_host> h = host_copy(x, n);
+    for (int i = 0; i < n; i++) {
+        std::printf("%.3e\t", h[i].re);
+        if ((i + 1) % N == 0) std::printf("\n");
+    }
+}
+void Print(double *x, int n)
+{
+    std::vector<double> h(n);
+    hck(hipMemcpy(h.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost), "Print copy");
+    for (int i = 0; i < n; i++) {
+        std::printf("%.3e\t", h[i]);
+        if ((i + 1) % N == 0) std::printf("\n");
+    }
+}
+void Print(fftw_complex *x)
+{
+    const std::vector<cd_host> h = host_copy(x, (long)N * N);
+    for (int i = 0; i < N; i++) {
+        for (int j = 0; j < N; j++) std::printf("(%.3e,%.3e)\t", h[i * N + j].re, h[i * N + j].im);
+        std::printf("\n");
+    }
+}
+void Print(double *x)
+{
+    std::vector<double> h((size_t)N * N);
+    hck(hipMemcpy(h.data(), x, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "Print copy");
+    for (int i = 0; i < N; i++) {
+        for (int j = 0; j < N; j++) std::printf("%.3e\t", h[i * N + j]);
+        std::printf("\n");
+    }
+}
+void Printf_begin_end(fftw_complex *x)
+{
+    const std::vector<cd_host> h = host_copy(x, CplxDofs_local);
+    std::printf("\n begin:  %.3e \t", h[0].re);
+    std::printf("end:  %.3e \n", h[CplxDofs_local - 1].re);
 }
 
 // ----------------------------------------------------------------- Memfree (Memfree.cpp:5-73)

@@ -68,6 +68,32 @@ __global__ void __launch_bounds__(256) k_dot(const T *__restrict__ a, const T *_
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
+// get_Ene's two sums as the reference forms them (LPsrc.cpp:11-35): the density
+// -Tau/2 phi^2 - Gamma/3 phi^3 + phi^4/4 of the real field, and G |u_F|^2 over
+// the full complex spectrum u_F (N^3 entries, x-major), with G looked up by |n|^2.
+template <typename T>
+__global__ void __launch_bounds__(256) k_energy_uf(const T *__restrict__ phi, const C2<T> *__restrict__ uF,
+                                                  const double *__restrict__ G, int N, long n, double Tau,
+                                                  double Gamma, double *__restrict__ part)
+{
+    __shared__ double red[2 * 16];
+    double s[2] = {0.0, 0.0};
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) {
+        const double u = (double)phi[l];
+        const double u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+        s[0] += -0.5 * Tau * u2 - Gamma * u3 / 3 + 0.25 * u4;
+        const int k = (int)(l % N), j = (int)((l / N) % N), i = (int)(l / ((long)N * N));
+        const int a = kfold(i, N), b = kfold(j, N), cz = kfold(k, N);
+        const double re = (double)uF[l].x, im = (double)uF[l].y;
+        s[1] += (re * re + im * im) * G[a * a + b * b + cz * cz];
+    }
+    block_reduce<2, 0u>(s, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s[0];
+        part[gridDim.x + blockIdx.x] = s[1];
+    }
+}
+
 template <typename T>
 __global__ void k_lin2(T *r, double a1, const T *F1, double a2, const T *F2, long n)
 {
@@ -541,6 +567,26 @@ int spm_energy(spm_ctx *h, const void *phi, double *E)
     double v[2];
     if ((rc = spm_read_scalars(h, c.scal + SC_E1, 2, v))) return rc;
     *E = 0.5 * v[0] + v[1]; // E1 + E2 (LPsrc.cpp:35-36)
+    return SPM_OK;
+}
+
+int spm_energy_uf(spm_ctx *h, const void *phi, const void *u_F, double *E)
+{
+    CHECK_CTX(h);
+    if (!phi || !u_F || !E) return spm_fail(SPM_EINVAL, "spm_energy_uf: null argument");
+    double *part = c.slot(7);
+    if (c.prec == SPM_FP64)
+        hipLaunchKernelGGL(k_energy_uf<double>, dim3(DOT_BLOCKS), dim3(256), 0, c.stream, (const double *)phi,
+                           (const C2<double> *)u_F, c.Gtab, c.N, c.R, c.Tau, c.Gamma, part);
+    else
+        hipLaunchKernelGGL(k_energy_uf<float>, dim3(DOT_BLOCKS), dim3(256), 0, c.stream, (const float *)phi,
+                           (const C2<float> *)u_F, c.Gtab, c.N, c.R, c.Tau, c.Gamma, part);
+    launch_reduce(c, part, DOT_BLOCKS, 2, 0u, 1.0, c.scal + SC_TMP0);
+    int rc = spm_launch_status("spm_energy_uf");
+    if (rc) return rc;
+    double v[2];
+    if ((rc = spm_read_scalars(h, c.scal + SC_TMP0, 2, v))) return rc;
+    *E = 0.5 * v[1] + v[0] / (double)c.R; // E1 + E2 (LPsrc.cpp:24, 35-36)
     return SPM_OK;
 }
 

@@ -25,6 +25,9 @@ def test_compat_functions_match_oracle(N):
     assert os.path.exists(exe), "build the test program first (__graft_entry__.build())"
     out = _run([exe, str(N)])
     res = json.loads([l for l in out.splitlines() if l.startswith("{")][-1])
+    assert res["energy_uf_rel"] < 1e-12  # get_Ene reads u_F_local (LPsrc.cpp:27-35)
+    assert res["basic_ops"] < 1e-13  # BasicOperators.h:6-50 against host loops
+    assert res["copy_option_V"] == 0.0  # LOBPCG.cpp:489-520, all three overloads
     assert res["fft"] < 1e-15 and res["ifft"] < 1e-11
     assert res["energy_rel"] < 1e-12
     for key in ("ngrad", "hv", "precond", "tv", "natural", "minmax"):
@@ -91,3 +94,35 @@ def test_stage2_driver(seeds, tmp_path):
     assert res[0]["E_min"] < res[0]["E_saddle"]  # gradient flow from the saddle goes downhill
     vmin = np.loadtxt(tmp_path / "V0_Min_a.txt")
     assert vmin.shape == (3 * N ** 3,) and np.isfinite(vmin).all()
+
+
+@pytest.mark.parametrize("driver", ["critical_point_HISD", "critical_point_HISD_v", "critical_point",
+                                    "critical_point_v"])
+def test_saddle_drivers_match_oracle(driver):
+    """critical_point_HISD(_v) (HISD.cpp:13-83) and critical_point(_v) (GOSD.cpp:13-120)
+    through the compat layer, in the {110} cell at 16^3: u_R_local is the index-1
+    saddle Spring_Pairing reaches, option1 its lowest 4 modes.  The oracle runs
+    hiosd_lobpcg_sieh(_V0) with the driver's perturbation, directions, null space and
+    parameters from the same inputs.  Both must end at the same energy, and at the
+    same field to the driver's own tolerance; the drivers that read seed files
+    (FCC, HPC) find them in tests/golden/initData.
+
+    The HiSD drivers climb back to the saddle along its one unstable mode, which V
+    must match.  The GOSD drivers' null space V0 is option1's first three modes,
+    the saddle's unstable mode among them, so they relax to the ordered minimum
+    below it, where V ends in the (degenerate) translation modes: only the field
+    and the energy are unique there."""
+    exe = os.path.join(ROOT, "tests", "compat", "compat_parity")
+    out = _run([exe, "drivers", driver, "16"], cwd=os.path.join(ROOT, "tests", "golden"))
+    res = json.loads([l for l in out.splitlines() if l.startswith("{")][-1])
+    print(res, flush=True)
+    assert res["res_oracle"] < res["epsf"] and res["iters_oracle"] < res["maxiter"]
+    assert res["res"] < res["epsf"] * 1.01
+    assert res["x"] < max(1e-8, res["epsf"])
+    # at a point where ||grad F|| < epsf, E moves by at most about epsf |dx| with the field
+    assert abs(res["E"] - res["E_oracle"]) <= 1e-10 * abs(res["E_oracle"]) + res["epsf"] * res["x"]
+    if driver.startswith("critical_point_HISD"):
+        assert res["v"] < 1e-8
+        assert abs(res["E"] - res["E_S"]) < 1e-5 * res["E_S"]  # the saddle, to epsf 1e-7
+    else:
+        assert res["E"] < -2.4e-6  # the ordered minimum (-2.40342e-6)

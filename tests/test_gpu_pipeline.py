@@ -127,3 +127,24 @@ def test_saddle_mixed_vs_fp64(runs):
     E64 = r64["saddle"]["E"]
     assert abs(rmx["saddle"]["E"] - E64) <= 1e-3 * abs(E64)
     assert np.abs(rmx["phi1"] - r64["phi1"]).max() <= 1e-3 * np.abs(r64["phi1"]).max()
+
+
+@pytest.mark.parametrize("Nc", [64, 128])
+def test_cell_workflow_mixed_matches_fp64_at_converged_states(pipe, Nc):
+    """Both stages of the workflow in the {110} cell (pipeline.cell_workflow):
+    the index-1 saddle (Spring_Pairing to res 1e-9) and the ordered minimum below
+    it (gradient flow to ||grad F|| 1e-8), each reached by its tolerance in both
+    precisions.  Mixed (FP32 stages handing over to FP64) must give the FP64
+    energies to 1e-9 relative and the same Hessian eigenvalues to 1e-9, with
+    index 1 at the saddle and 0 at the minimum."""
+    print("[case] cell workflow fp64 vs mixed N=%d" % Nc, flush=True)
+    rep = {p: pipe.cell_workflow(Nc, p)[0] for p in ("fp64", "mixed")}
+    for p in ("fp64", "mixed"):
+        s, m = rep[p]["saddle"], rep[p]["minimum"]
+        assert s["res"] <= 1e-9 and s["iters"] < 30000 and s["index"] == 1, (p, s)
+        assert m["gn"] <= 1e-8 and m["index"] == 0 and m["E"] < 0, (p, m)
+    f, x = rep["fp64"], rep["mixed"]
+    assert x["saddle"]["iters_fp32"] > 0 and x["minimum"]["iters_fp32"] > 0  # the FP32 stages did the bulk
+    for stage in ("saddle", "minimum"):
+        assert abs(x[stage]["E"] - f[stage]["E"]) <= 1e-9 * abs(f[stage]["E"]), stage
+        assert np.abs(np.array(x[stage]["ritz"]) - np.array(f[stage]["ritz"])).max() < 1e-9, stage
