@@ -3,9 +3,12 @@
 // counterpart of the reference's FFTW_PATIENT planning and wisdom files
 // (My_mpi_fft_set_up, My_FFT.cpp:45-97).
 //
-// Cache: one line per measurement, "arch N precision members shape us_per_step",
-// appended to $SPM_TUNE_CACHE (default $HOME/.cache/spm-iqc/shapes.txt); the
-// last line for a key wins.  Members are bucketed to the next power of two.
+// Cache: one line per measurement, "arch N precision members shape us_per_step
+// table", appended to $SPM_TUNE_CACHE (default $HOME/.cache/spm-iqc/shapes.txt);
+// the last line for a key wins.  Members are bucketed to the next power of two.
+// "table" is a signature of N's shape table in this build: a line recorded with
+// another table (a rebuild that added, dropped or reordered shapes) never
+// matches, and the shape is measured again.
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
@@ -51,9 +54,27 @@ int bucket(int nmem)
     return b;
 }
 
-std::string key(const std::string &ar, int N, int prec, int nmem)
+// FNV-1a of N's shape table (the threads per pencil of every shape, in order)
+std::string table_sig(int N)
 {
-    return ar + " " + std::to_string(N) + " " + std::to_string(prec) + " " + std::to_string(bucket(nmem));
+    const int (*s)[3] = nullptr;
+    const int n = grid_shapes(N, &s);
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](int v) {
+        h ^= (unsigned)v;
+        h *= 1099511628211ull;
+    };
+    mix(n);
+    for (int k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) mix(s[k][a]);
+    char buf[24];
+    std::snprintf(buf, sizeof buf, "%016llx", h);
+    return buf;
+}
+
+std::string key(const std::string &ar, int N, int prec, int nmem, const std::string &sig)
+{
+    return ar + " " + std::to_string(N) + " " + std::to_string(prec) + " " + std::to_string(bucket(nmem)) + " " + sig;
 }
 
 void load_cache() // g_mu held
@@ -62,22 +83,24 @@ void load_cache() // g_mu held
     g_loaded = true;
     FILE *f = std::fopen(cache_path().c_str(), "r");
     if (!f) return;
-    char ar[64];
+    char line[256], ar[64], sig[32];
     int N, prec, nm, shape;
     double us;
-    while (std::fscanf(f, "%63s %d %d %d %d %lf", ar, &N, &prec, &nm, &shape, &us) == 6)
-        g_table[key(ar, N, prec, nm)] = shape;
+    while (std::fgets(line, sizeof line, f)) // lines without a table signature are skipped
+        if (std::sscanf(line, "%63s %d %d %d %d %lf %31s", ar, &N, &prec, &nm, &shape, &us, sig) == 7)
+            g_table[key(ar, N, prec, nm, sig)] = shape;
     std::fclose(f);
 }
 
 void record(const std::string &ar, int N, int prec, int nmem, int shape, double us) // g_mu held
 {
-    g_table[key(ar, N, prec, nmem)] = shape;
+    const std::string sig = table_sig(N);
+    g_table[key(ar, N, prec, nmem, sig)] = shape;
     const std::string path = cache_path();
     for (size_t k = path.find('/', 1); k != std::string::npos; k = path.find('/', k + 1))
         (void)mkdir(path.substr(0, k).c_str(), 0755); // mkdir -p of the parent; fopen reports failure
     if (FILE *f = std::fopen(path.c_str(), "a")) {
-        std::fprintf(f, "%s %d %d %d %d %.3f\n", ar.c_str(), N, prec, bucket(nmem), shape, us);
+        std::fprintf(f, "%s %d %d %d %d %.3f %s\n", ar.c_str(), N, prec, bucket(nmem), shape, us, sig.c_str());
         std::fclose(f);
     }
 }
@@ -161,8 +184,9 @@ int spm_resolve_shape(const spm_params *p, int nmem, int *shape)
     {
         std::lock_guard<std::mutex> lk(g_mu);
         load_cache();
-        auto it = g_table.find(key(arch(), p->N, p->precision, nmem));
-        if (it != g_table.end()) {
+        auto it = g_table.find(key(arch(), p->N, p->precision, nmem, table_sig(p->N)));
+        const int (*s)[3] = nullptr;
+        if (it != g_table.end() && it->second >= 0 && it->second < grid_shapes(p->N, &s)) {
             *shape = it->second;
             return SPM_OK;
         }

@@ -95,15 +95,31 @@ def prepare_states(reqs, verbose=False):
     return states
 
 
-def footprint(r, prec_bytes=4):
-    """Device bytes a worker request holds: fields, FFT work spectra, block state."""
+# per context: the Gram partials and block coefficients of the block operations
+# (2048 blocks x a 32 x 32 tile of doubles, plus 4 x 4096 doubles)
+CTX_SCRATCH = 8 * (2048 * 32 * 32 + 4 * 4096)
+
+
+def footprint(r):
+    """Device bytes a worker request holds at its peak: the context's two work
+    spectra and scratch, its solver workspace, and the fields the request keeps
+    (peak over the stages: the mixed requests' FP64 polish holds FP64 copies)."""
     N = r["N"]
     R = N ** 3
-    spec = N * N * (N // 2 + 1 + 16) * 2 * prec_bytes
+
+    def ctx(b):
+        return 2 * N * N * (N // 2 + 1 + 16) * 2 * b + CTX_SCRATCH
+
     if r["kind"] == "modes":
-        k = 8
-        return 2 * spec + (7 * k + 1) * R * prec_bytes + k * R * 8
-    return 2 * spec + 6 * R * prec_bytes + 4 * R * 8
+        k = 8  # workspace U, W, Y, Vp, Up, T (6k fields), the block V (FP64 outside), phi
+        fp32 = ctx(4) + (6 * k + 2) * R * 4 + k * R * 8
+        fp64 = ctx(8) + (6 * k + 1) * R * 8 + k * R * 8
+        return max(fp32, fp64)
+    # saddle: two replicas, d and the kept spectrum (workspace), ngrad scratch; the FP64
+    # replicas made from the FP32 ones, and phi, v1
+    fp32 = ctx(4) + 4 * R * 4 + N * N * (N // 2 + 17) * 2 * 4 + 4 * R * 8
+    fp64 = ctx(8) + 4 * R * 8 + N * N * (N // 2 + 17) * 2 * 8 + 2 * R * 8
+    return max(fp32, fp64)
 
 
 # ---------------------------------------------------------------------------

@@ -148,19 +148,21 @@ def test_gradflow_fp64(spm, po, seeds, torch, N, kind, steps):
     assert np.abs(out - uR.real).max() <= 1e-10
 
 
-def test_config1_64_500_steps(spm, po, seeds, torch):
+@pytest.mark.parametrize("kscale", [1.0, 2.0 * np.pi])
+def test_config1_64_500_steps(spm, po, seeds, torch, kscale):
     """The 64^3 workload of the benchmark configurations: an i.i.d. N(0, 0.05^2)
     seed (seed 0, mean removed), 500 get_min_u_F steps (LPsrc.cpp:40-117) with
     E and ||grad F|| logged at every step, against the oracle (default step:
-    two passes at 64^3)."""
-    print("[case] config 1: 64^3, 500 steps", flush=True)
+    two passes at 64^3), on the reference's k = n / L grid and on the
+    k = 2 pi n / L grid the configuration names."""
+    print("[case] config 1: 64^3, 500 steps, kscale %g" % kscale, flush=True)
     N, steps = 64, 500
     phi = seeds.gaussian(N, 0.05, 0)
-    o = po.Oracle(N)
+    o = po.Oracle(N, kscale=kscale)
     uR = phi.astype(np.complex128)
     uF = o.my_fft(uR)
     it, E, res, gn = o.get_min_u_F(uR, uF, steps, -1.0, gnorm=True)
-    with spm.Grid(N, 0) as g:
+    with spm.Grid(N, 0, kscale=kscale) as g:
         pt = torch.as_tensor(phi).cuda()
         itg, hist = g.gradflow(pt, steps, -1.0, 0.0, check_every=64)
         out = _np(pt)

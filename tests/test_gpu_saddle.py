@@ -130,3 +130,24 @@ def test_critical_point_hisd_reaches_the_same_saddle(spm, po, torch, N):
                 if len(climbed) == 1:
                     _certify(po, g, torch, N, x, res, 1e-10, oracle_index=(N == 32))
     assert climbed, "no start climbed to the saddle"
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_lobpcg_block8_256_closed_form(spm, torch, prec):
+    """Block-8 LOBPCG (hiosd_lobpcg_inip, LOBPCG.cpp:207-487) at 256^3, the
+    configuration's block size: at phi = 0 in the {110} cell the Hessian is
+    G(k) - Tau on each Fourier mode, and its lowest eigenvalue G(1) - Tau is
+    12-fold (|n|^2 = 2), with the next shell (|n|^2 = 1) 0.07 above.  All eight
+    Ritz values must equal it, in FP64 fields and in FP32 fields (FP64 Gram,
+    Cholesky-QR and Rayleigh-Ritz)."""
+    print("[case] block-8 LOBPCG 256^3 prec=%d" % prec, flush=True)
+    N = 256
+    with spm.Grid(N, prec, L=L) as g:
+        phi = torch.zeros((N,) * 3, dtype=g.dtype, device="cuda")
+        start = np.random.default_rng(3).uniform(-1, 1, (8, N, N, N))
+        V = g.lobpcg_init(torch.as_tensor(start).to(g.dtype).cuda())
+        del start
+        for _ in range(3):
+            ritz = g.lobpcg(phi, V, 21)
+    lam = spm.get_GK(1.0) - TAU
+    assert np.abs(ritz - lam).max() < (1e-12 if prec == 0 else 1e-9), ritz - lam
