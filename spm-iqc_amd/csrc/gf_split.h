@@ -75,6 +75,8 @@ template <typename T, int N> struct SJ {
         kg = s;
         return P < DH * KO;
     }
+    // a pass-2 block with no valid column (kz of all its columns past N/2)
+    __device__ static bool cblock_empty(bool real_blk, int ko, int kg) { return !real_blk || ko * KT + KB * kg >= NH; }
     // U: the state of one pass-2 block is one contiguous chunk [l][i][ka]
     // (digit k2 at l); r = i KB + ka is the pass-2 thread of the line role
     __host__ __device__ static constexpr long uof(int h, int ko, int kg, int l, int r)
@@ -181,7 +183,10 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     constexpr int TPP = S::RTPP, E = N / TPP, NP = S::NP, S1 = S::S1, DH = S::DH, DL = S::DL, KT = S::KT;
     constexpr int NR = 2 * NP;
     constexpr int RS = N / 2 + 8;                      // row stride of the row buffer (reals)
-    constexpr int RSTR = pencil_slots<N, true>() + 1;  // pencil exchange stride (reals)
+    // Pencil exchange stride (reals), = 16 mod 32: the 4 pencils of a wave then
+    // sit 16 banks apart, and the first stage's writes (element 16 t + q at
+    // padded slot 17 t + q) of the 64 lanes fall in distinct banks.
+    constexpr int RSTR = pencil_slots<N, true>() + (48 - pencil_slots<N, true>() % 32) % 32;
     constexpr int NL = NR * RS > NP * RSTR ? NR * RS : NP * RSTR;
     __shared__ __align__(16) T lds[NL];
     __shared__ T nyq[NR];            // kz = N/2 entry of each row (real)
@@ -427,6 +432,13 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
     load_twiddles<T, N>(stw, tw);
     int h, ko, kg;
     const bool real_blk = S::cblock(blockIdx.x, h, ko, kg);
+    double acc[SOp::NRED > 0 ? SOp::NRED : 1];
+#pragma unroll
+    for (int q = 0; q < (SOp::NRED > 0 ? SOp::NRED : 1); ++q) acc[q] = 0.0;
+    if (S::cblock_empty(real_blk, ko, kg)) { // nothing to read or write: zero partial sums only
+        write_partials<SOp::NRED, SOp::MAXM>(acc, red, part);
+        return;
+    }
     // line role: thread r = (ia, ka) holds the DL values l of (i = ia, ki = KB kg + ka)
     const int r = threadIdx.x, ka = r % KB, ia = r / KB;
     const int kia = KB * kg + ka;
@@ -440,9 +452,6 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
     C2<T> v[E];
     static_assert(E == DL, "one register array serves both roles");
     C2<T> pre[SOp::PRELOAD ? DL : 1]; // state, loaded at the start: its latency overlaps the transforms
-    double acc[SOp::NRED > 0 ? SOp::NRED : 1];
-#pragma unroll
-    for (int q = 0; q < (SOp::NRED > 0 ? SOp::NRED : 1); ++q) acc[q] = 0.0;
     if constexpr (FWD) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = vb ? src[wbase + (long)(tb + TPP * m) * S::ISTR] : C2<T>{0, 0};
