@@ -193,15 +193,15 @@ __global__ void __launch_bounds__(256) k_gram_small(FieldPtrs X, int ma, FieldPt
 // block: thread group g (16 groups of 64) sums the blocks b = g (mod 16) in
 // order, then the 16 group sums combine by a fixed pairwise tree, so the
 // result does not depend on timing.
-__global__ void __launch_bounds__(1024) k_gram_reduce(const double *__restrict__ part, int nblk, double scale,
-                                                     double *__restrict__ out)
+__global__ void __launch_bounds__(1024) k_gram_reduce(const double *__restrict__ part, int nblk, long epb,
+                                                     double scale, double *__restrict__ out)
 {
     __shared__ double sh[16][64];
     const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int e = blockIdx.x * 64 + el;
     double s = 0.0;
 #pragma unroll 8
-    for (int b = g; b < nblk; b += 16) s += part[(long)b * GRAM_TILE * GRAM_TILE + e];
+    for (int b = g; b < nblk; b += 16) s += part[(long)b * epb + e];
     sh[g][el] = s;
     __syncthreads();
 #pragma unroll
@@ -212,55 +212,199 @@ __global__ void __launch_bounds__(1024) k_gram_reduce(const double *__restrict__
     if (g == 0) out[e] = sh[0][el] * scale;
 }
 
+// The Gram of a Rayleigh-Ritz basis X (mx <= 16 TA fields) against [X, Y] (Y:
+// my fields, mx + my <= 16 TB), and the means of [X, Y] (ONES: row mx of the
+// row operand is the constant 1), in one pass that reads each field once.
+// Column c < mx is X_c, which is row c of the same tile's row operand (tile
+// rows and tile columns line up in the lanes), so it is not loaded again.
+// The MFMA layout and the fixed summation order are those of k_gram; the
+// partials of block b are the TA 16 x TB 16 entries at part + b (TA 16)(TB 16).
+template <typename T, int TA, int TB>
+__global__ void __launch_bounds__(256) k_gram_xy(FieldPtrs X, int mx, FieldPtrs Y, int my, int ones, long R,
+                                                double *__restrict__ part)
+{
+    static_assert(TB >= TA, "the columns start with X");
+    constexpr int CW = TB * 16;
+    __shared__ double sh[4][TA * TB * 4][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    d4 acc[TA][TB];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) acc[ta][tb] = d4{0.0, 0.0, 0.0, 0.0};
+    const T *xa[TA], *yb[TB];
+    bool one[TA], reuse[TB];
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+        const int r = ta * 16 + row;
+        xa[ta] = r < mx ? (const T *)X.p[r] : nullptr;
+        one[ta] = ones && r == mx;
+    }
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) {
+        const int cc = tb * 16 + row;
+        reuse[tb] = tb < TA && cc < mx;
+        yb[tb] = (cc >= mx && cc - mx < my) ? (const T *)Y.p[cc - mx] : nullptr;
+    }
+    auto ld4 = [&](const T *f, long r0, double (&v)[4]) {
+        if constexpr (sizeof(T) == 8) {
+            const double2 u = *(const double2 *)(f + r0), w = *(const double2 *)(f + r0 + 2);
+            v[0] = u.x; v[1] = u.y; v[2] = w.x; v[3] = w.y;
+        } else {
+            const float4 u = *(const float4 *)(f + r0);
+            v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+        }
+    };
+    const long nchunk = R / 16;
+    auto load = [&](long q, double (&a)[TA][4], double (&b)[TB][4]) {
+        const long r0 = q * 16 + 4 * kq;
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta) {
+            if (xa[ta]) ld4(xa[ta], r0, a[ta]);
+            else a[ta][0] = a[ta][1] = a[ta][2] = a[ta][3] = one[ta] ? 1.0 : 0.0;
+        }
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+            if (yb[tb]) ld4(yb[tb], r0, b[tb]);
+            else b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = 0.0;
+        }
+    };
+    const long step = (long)gridDim.x * 4;
+    long q = (long)blockIdx.x * 4 + wid;
+    double a[TA][4], b[TB][4];
+    if (q < nchunk) load(q, a, b);
+    for (; q < nchunk; q += step) {
+        double an[TA][4], bn[TB][4];
+        if (q + step < nchunk) load(q + step, an, bn);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int tb = 0; tb < TB; ++tb) {
+                const double bv = (tb < TA && reuse[tb]) ? a[tb < TA ? tb : 0][s] : b[tb][s];
+#pragma unroll
+                for (int ta = 0; ta < TA; ++ta)
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], bv, acc[ta][tb], 0, 0, 0);
+            }
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[ta][s] = an[ta][s];
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) b[tb][s] = bn[tb][s];
+    }
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sh[wid][(ta * TB + tb) * 4 + v][lane] = acc[ta][tb][v];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int e = (ta * TB + tb) * 4 + v;
+                    const double sum = ((sh[0][e][lane] + sh[1][e][lane]) + sh[2][e][lane]) + sh[3][e][lane];
+                    const int ai = ta * 16 + kq + 4 * v, bj = tb * 16 + row;
+                    part[(long)blockIdx.x * (TA * 16) * CW + ai * CW + bj] = sum;
+                }
+    }
+}
+
 constexpr int COMBINE_NO = 8; // outputs per k_combine launch
 
-// Operands of one k_combine launch, passed by value (2.4 KB of kernel
-// arguments, read with uniform scalar loads): no upload, so a combine queues
-// behind earlier work without draining the stream.
-struct CombineArgs {
-    const void *in[BLK_MAX];
-    void *out[COMBINE_NO];
-    double coef[BLK_MAX * COMBINE_NO]; // coef[a * COMBINE_NO + o]
-    double shift[COMBINE_NO];
+// Operands of one k_combine launch, passed by value (read with uniform scalar
+// loads): no upload, so a combine queues behind earlier work without draining
+// the stream.  Two sizes: up to 32 inputs and 8 outputs (2.4 KB of kernel
+// arguments), and up to 24 inputs and 16 outputs (3.5 KB).
+template <int NI_, int NO_> struct CombineArgsT {
+    static constexpr int NI = NI_, NO = NO_;
+    const void *in[NI];
+    void *out[NO];
+    double coef[NI * NO]; // coef[a * NO + o]
+    double shift[NO];
+};
+using CombineArgs = CombineArgsT<BLK_MAX, COMBINE_NO>;
+using CombineArgsWide = CombineArgsT<24, 16>;
+
+// out_o = sum_a coef[a][o] in_a + shift[o], o < NO: PT consecutive points per
+// thread (4, or 2 for more than 8 outputs, which keeps the accumulators at 64
+// registers), all inputs read once per point (so an output may alias an input).
+template <typename T, int PT> struct PtVec;
+template <> struct PtVec<double, 4> {
+    static __device__ __forceinline__ void ld(const void *p, long i, double (&x)[4])
+    {
+        const double2 u = ((const double2 *)p)[2 * i], w = ((const double2 *)p)[2 * i + 1];
+        x[0] = u.x; x[1] = u.y; x[2] = w.x; x[3] = w.y;
+    }
+    static __device__ __forceinline__ void st(void *p, long i, const double (&x)[4])
+    {
+        ((double2 *)p)[2 * i] = double2{x[0], x[1]};
+        ((double2 *)p)[2 * i + 1] = double2{x[2], x[3]};
+    }
+};
+template <> struct PtVec<double, 2> {
+    static __device__ __forceinline__ void ld(const void *p, long i, double (&x)[2])
+    {
+        const double2 u = ((const double2 *)p)[i];
+        x[0] = u.x; x[1] = u.y;
+    }
+    static __device__ __forceinline__ void st(void *p, long i, const double (&x)[2])
+    {
+        ((double2 *)p)[i] = double2{x[0], x[1]};
+    }
+};
+template <> struct PtVec<float, 4> {
+    static __device__ __forceinline__ void ld(const void *p, long i, double (&x)[4])
+    {
+        const float4 u = ((const float4 *)p)[i];
+        x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w;
+    }
+    static __device__ __forceinline__ void st(void *p, long i, const double (&x)[4])
+    {
+        ((float4 *)p)[i] = float4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
+    }
+};
+template <> struct PtVec<float, 2> {
+    static __device__ __forceinline__ void ld(const void *p, long i, double (&x)[2])
+    {
+        const float2 u = ((const float2 *)p)[i];
+        x[0] = u.x; x[1] = u.y;
+    }
+    static __device__ __forceinline__ void st(void *p, long i, const double (&x)[2])
+    {
+        ((float2 *)p)[i] = float2{(float)x[0], (float)x[1]};
+    }
 };
 
-// out_o = sum_a coef[a][o] in_a + shift[o], o < NO: four consecutive points
-// per thread, all inputs read once per point (so an output may alias an input).
-template <typename T, int NO>
-__global__ void __launch_bounds__(256) k_combine(const CombineArgs args, int nin, long R4)
+template <typename T, int NO, int PT, class Args>
+__global__ void __launch_bounds__(256) k_combine(const Args args, int nin, long RP)
 {
-    for (long r4 = (long)blockIdx.x * blockDim.x + threadIdx.x; r4 < R4; r4 += (long)gridDim.x * blockDim.x) {
-        double acc[NO][4];
+    static_assert(NO <= Args::NO, "outputs");
+    for (long rp = (long)blockIdx.x * blockDim.x + threadIdx.x; rp < RP; rp += (long)gridDim.x * blockDim.x) {
+        double acc[NO][PT];
 #pragma unroll
         for (int o = 0; o < NO; ++o)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc[o][k] = args.shift[o];
+            for (int k = 0; k < PT; ++k) acc[o][k] = args.shift[o];
         for (int a = 0; a < nin; ++a) {
-            double x[4];
-            if constexpr (sizeof(T) == 4) {
-                const float4 u = ((const float4 *)args.in[a])[r4];
-                x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w;
-            } else {
-                const double2 u = ((const double2 *)args.in[a])[2 * r4], w = ((const double2 *)args.in[a])[2 * r4 + 1];
-                x[0] = u.x; x[1] = u.y; x[2] = w.x; x[3] = w.y;
-            }
+            double x[PT];
+            PtVec<T, PT>::ld(args.in[a], rp, x);
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
-                const double cf = args.coef[a * COMBINE_NO + o];
+                const double cf = args.coef[a * Args::NO + o];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) acc[o][k] += cf * x[k];
+                for (int k = 0; k < PT; ++k) acc[o][k] += cf * x[k];
             }
         }
 #pragma unroll
-        for (int o = 0; o < NO; ++o) {
-            if constexpr (sizeof(T) == 4) {
-                ((float4 *)args.out[o])[r4] = float4{(float)acc[o][0], (float)acc[o][1], (float)acc[o][2],
-                                                     (float)acc[o][3]};
-            } else {
-                ((double2 *)args.out[o])[2 * r4] = double2{acc[o][0], acc[o][1]};
-                ((double2 *)args.out[o])[2 * r4 + 1] = double2{acc[o][2], acc[o][3]};
-            }
-        }
+        for (int o = 0; o < NO; ++o) PtVec<T, PT>::st(args.out[o], rp, acc[o]);
     }
 }
 
@@ -291,9 +435,12 @@ __global__ void k_sums_reduce(const double *__restrict__ part, int nblk, int m, 
 
 // ---------------------------------------------------------------- host side
 
+constexpr int XY_ROWS = 32, XY_COLS = 48; // k_gram_xy: at most 2 x 3 tiles
+constexpr size_t GRAM_PART_EPB = (size_t)XY_ROWS * XY_COLS; // partials per block, k_gram and k_gram_xy
+
 static int ensure_bscr(spm_ctx *h)
 {
-    const size_t need = sizeof(double) * ((size_t)GRAM_BLOCKS * GRAM_TILE * GRAM_TILE + 4 * 4096);
+    const size_t need = sizeof(double) * ((size_t)GRAM_BLOCKS * GRAM_PART_EPB + 4 * 4096);
     if (h->bscr_bytes >= need) return SPM_OK;
     if (h->bscr) {
         HIPCHK(hipStreamSynchronize(h->c.stream));
@@ -306,7 +453,8 @@ static int ensure_bscr(spm_ctx *h)
 }
 // scratch regions inside bscr
 static double *gram_part(spm_ctx *h) { return h->bscr; }
-static double *small_dev(spm_ctx *h) { return h->bscr + (size_t)GRAM_BLOCKS * GRAM_TILE * GRAM_TILE; }
+static double *small_dev(spm_ctx *h) { return h->bscr + (size_t)GRAM_BLOCKS * GRAM_PART_EPB; }
+double *block_small(spm_ctx *h) { return ensure_bscr(h) == SPM_OK ? small_dev(h) + 2048 : nullptr; }
 
 int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, int mb, double *out)
 {
@@ -356,7 +504,7 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     double *dout = small_dev(h);
     // rows 0 .. ma-1 of the 32-wide partials: (ma * 32 + 63) / 64 groups of 64 entries
     hipLaunchKernelGGL(k_gram_reduce, dim3((ma * GRAM_TILE + 63) / 64), dim3(1024), 0, c.stream, part, nblk,
-                       1.0 / (double)c.R, dout);
+                       (long)GRAM_TILE * GRAM_TILE, 1.0 / (double)c.R, dout);
     RCHK(spm_launch_status("block_gram"));
     std::vector<double> full(GRAM_TILE * GRAM_TILE);
     HIPCHK(hipMemcpyAsync(full.data(), dout, sizeof(double) * ma * GRAM_TILE, hipMemcpyDeviceToHost, c.stream));
@@ -366,28 +514,109 @@ int block_gram(spm_ctx *h, const void *const *X, int ma, const void *const *Y, i
     return SPM_OK;
 }
 
+int block_gram_xy(spm_ctx *h, const void *const *X, int mx, const void *const *Y, int my, double *A, double *B,
+                  double *mean)
+{
+    Ctx &c = h->c;
+    if (mx < 1 || my < 0 || mx + 1 > XY_ROWS || mx + my > XY_COLS) return spm_fail(SPM_EINVAL, "block_gram_xy: sizes");
+    RCHK(ensure_bscr(h));
+    FieldPtrs xp{}, yp{};
+    for (int a = 0; a < mx; ++a) xp.p[a] = X[a];
+    for (int b = 0; b < my; ++b) yp.p[b] = Y[b];
+    const int ta = (mx + 1 + 15) / 16, tb = std::max(ta, (mx + my + 15) / 16);
+    const int nblk = (int)std::min<long>(GRAM_BLOCKS, std::max<long>(1, c.R / 64));
+    double *part = gram_part(h);
+#define XY_LAUNCH(T, A_, B_)                                                                                   \
+    hipLaunchKernelGGL((k_gram_xy<T, A_, B_>), dim3(nblk), dim3(256), 0, c.stream, xp, mx, yp, my, 1, c.R, part)
+#define XY_BY_TILES(T)                                                                                         \
+    if (ta == 1 && tb == 1) XY_LAUNCH(T, 1, 1);                                                                \
+    else if (ta == 1 && tb == 2) XY_LAUNCH(T, 1, 2);                                                           \
+    else if (ta == 1) XY_LAUNCH(T, 1, 3);                                                                      \
+    else if (tb == 2) XY_LAUNCH(T, 2, 2);                                                                      \
+    else XY_LAUNCH(T, 2, 3);
+    if (c.prec == SPM_FP64) {
+        XY_BY_TILES(double)
+    } else {
+        XY_BY_TILES(float)
+    }
+#undef XY_BY_TILES
+#undef XY_LAUNCH
+    const int cw = tb * 16;
+    const long used = (long)(mx + 1) * cw;
+    double *dout = small_dev(h);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((used + 63) / 64)), dim3(1024), 0, c.stream, part, nblk,
+                       (long)ta * 16 * cw, 1.0 / (double)c.R, dout);
+    RCHK(spm_launch_status("block_gram_xy"));
+    std::vector<double> full(used);
+    HIPCHK(hipMemcpyAsync(full.data(), dout, sizeof(double) * used, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    for (int a = 0; a < mx; ++a) {
+        for (int b = 0; b < mx; ++b) B[a * mx + b] = full[a * cw + b];
+        for (int b = 0; b < my; ++b) A[a * my + b] = full[a * cw + mx + b];
+    }
+    for (int b = 0; b < mx; ++b) mean[b] = full[mx * cw + b];
+    return SPM_OK;
+}
+
+template <class Args>
+static Args combine_args(const void *const *in, int nin, void *const *out, int no, int i0, int nout,
+                         const double *coef, const double *shift)
+{
+    Args ca{};
+    for (int a = 0; a < nin; ++a) ca.in[a] = in[a];
+    for (int o = 0; o < no; ++o) {
+        ca.out[o] = out[i0 + o];
+        ca.shift[o] = shift ? shift[i0 + o] : 0.0;
+        for (int a = 0; a < nin; ++a) ca.coef[a * Args::NO + o] = coef[a * nout + i0 + o];
+    }
+    return ca;
+}
+
 int block_combine(spm_ctx *h, const void *const *in, int nin, void *const *out, int nout, const double *coef,
                   const double *shift)
 {
     Ctx &c = h->c;
     if (nin < 1 || nout < 1 || nin > BLK_MAX || nout > BLK_MAX) return spm_fail(SPM_EINVAL, "block_combine");
+    // R = N^3, N >= 4: a multiple of 4
+    auto nblocks = [&](long rp) { return (int)std::min<long>(2048, (rp + 255) / 256); };
+    if (nout > COMBINE_NO && nout <= CombineArgsWide::NO && nin <= CombineArgsWide::NI) {
+        // one launch: every input read once
+        const CombineArgsWide ca = combine_args<CombineArgsWide>(in, nin, out, nout, 0, nout, coef, shift);
+        const long RP = c.R / 2;
+#define WIDE_LAUNCH(T, K)                                                                                      \
+    hipLaunchKernelGGL((k_combine<T, K, 2, CombineArgsWide>), dim3(nblocks(RP)), dim3(256), 0, c.stream, ca, nin, RP)
+#define WIDE_BY_NO(T)                                                                                          \
+    switch (nout) {                                                                                            \
+    case 9: WIDE_LAUNCH(T, 9); break;                                                                          \
+    case 10: WIDE_LAUNCH(T, 10); break;                                                                        \
+    case 11: WIDE_LAUNCH(T, 11); break;                                                                        \
+    case 12: WIDE_LAUNCH(T, 12); break;                                                                        \
+    case 13: WIDE_LAUNCH(T, 13); break;                                                                        \
+    case 14: WIDE_LAUNCH(T, 14); break;                                                                        \
+    case 15: WIDE_LAUNCH(T, 15); break;                                                                        \
+    default: WIDE_LAUNCH(T, 16); break;                                                                        \
+    }
+        if (c.prec == SPM_FP64) {
+            WIDE_BY_NO(double)
+        } else {
+            WIDE_BY_NO(float)
+        }
+#undef WIDE_BY_NO
+#undef WIDE_LAUNCH
+        return spm_launch_status("block_combine");
+    }
     constexpr int NO = COMBINE_NO;
     if (nout > NO) // later launches would read inputs that earlier ones overwrote
         for (int a = 0; a < nin; ++a)
             for (int i = 0; i < nout; ++i)
                 if (in[a] == out[i]) return spm_fail(SPM_EINVAL, "block_combine: in place with more than 8 outputs");
-    const long R4 = c.R / 4; // N^3, N >= 4: a multiple of 4
-    const int nb = (int)std::min<long>(2048, (R4 + 255) / 256);
+    const long R4 = c.R / 4;
+    const int nb = nblocks(R4);
     for (int i0 = 0; i0 < nout; i0 += NO) {
         const int no = std::min(NO, nout - i0);
-        CombineArgs ca{};
-        for (int a = 0; a < nin; ++a) ca.in[a] = in[a];
-        for (int o = 0; o < no; ++o) {
-            ca.out[o] = out[i0 + o];
-            ca.shift[o] = shift ? shift[i0 + o] : 0.0;
-            for (int a = 0; a < nin; ++a) ca.coef[a * NO + o] = coef[a * nout + i0 + o];
-        }
-#define COMBINE_LAUNCH(T, K) hipLaunchKernelGGL((k_combine<T, K>), dim3(nb), dim3(256), 0, c.stream, ca, nin, R4)
+        const CombineArgs ca = combine_args<CombineArgs>(in, nin, out, no, i0, nout, coef, shift);
+#define COMBINE_LAUNCH(T, K)                                                                                   \
+    hipLaunchKernelGGL((k_combine<T, K, 4, CombineArgs>), dim3(nb), dim3(256), 0, c.stream, ca, nin, R4)
 #define COMBINE_BY_NO(T)                                                                                       \
     switch (no) {                                                                                              \
     case 1: COMBINE_LAUNCH(T, 1); break;                                                                       \
@@ -448,13 +677,14 @@ int field_zero(spm_ctx *h, void *dst)
 // sequential deflation: column i is dropped when its remaining norm is <= tol.
 // Returns Rinv (upper triangular, dropped columns zero) so that Q R^{-1} has
 // orthonormal kept columns.
-static void chol_inv(int n, const double *S, double tol, bool deflate, bool *keep, double *Rinv)
+void chol_inv(int n, const double *S, double tol, bool deflate, bool *keep, double *Rinv, double *piv)
 {
     std::vector<double> Rm(n * n, 0.0);
     for (int i = 0; i < n; ++i) {
         keep[i] = true;
         double d = S[i * n + i];
         for (int l = 0; l < i; ++l) d -= Rm[l * n + i] * Rm[l * n + i];
+        if (piv) piv[i] = d;
         if (deflate ? !(d > tol * tol) : !(d > 0.0)) {
             keep[i] = false;
             continue;
@@ -517,7 +747,7 @@ int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb
     // CholQR with the reference's deflation, then a second CholQR on the kept fields
     std::vector<double> S(nq * nq), Rinv(nq * nq);
     RCHK(block_gram(h, qin.data(), nq, qin.data(), nq, S.data()));
-    chol_inv(nq, S.data(), tol, tol >= 0.0, keep, Rinv.data());
+    chol_inv(nq, S.data(), tol, tol >= 0.0, keep, Rinv.data(), nullptr);
     RCHK(block_combine(h, qin.data(), nq, Q, nq, Rinv.data(), nullptr));
     std::vector<int> kept;
     for (int i = 0; i < nq; ++i)
@@ -533,7 +763,7 @@ int block_ortho(spm_ctx *h, void *const *Q, int nq, const void *const *B, int nb
         std::vector<double> S2(nk * nk), R2(nk * nk);
         bool k2[BLK_MAX];
         RCHK(block_gram(h, kin.data(), nk, kin.data(), nk, S2.data()));
-        chol_inv(nk, S2.data(), -1.0, false, k2, R2.data());
+        chol_inv(nk, S2.data(), -1.0, false, k2, R2.data(), nullptr);
         RCHK(block_combine(h, kin.data(), nk, kout.data(), nk, R2.data(), nullptr));
     }
     return SPM_OK;

@@ -194,22 +194,49 @@ struct GridImpl final : GridIface {
                           nullptr);
     }
 
+    void hv_dot(Ctx &c, const void *phi, const void *v, void *hvo, double *dot) override
+    {
+        const OpHvOut<T, true> out{c.Tau, c.Gamma, (const T *)phi, (const T *)v, c.scal + SC_MEAN_CV, (T *)hvo};
+        if constexpr (N == 64 || N == 128 || N == 256) {
+            if (two_pass(c)) {
+                sj_round_trip(
+                    c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0),
+                    [&] { launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV); },
+                    SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, out, c.slot(1));
+                launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, dot);
+                return;
+            }
+        }
+        rows<false, true>(c, v, c.W, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0));
+        launch_reduce(c, c.slot(0), c.nb_rows, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV);
+        cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, nullptr);
+        cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
+        rows<true, false>(c, c.W, nullptr, out, c.slot(1));
+        launch_reduce(c, c.slot(1), c.nb_rows, 1, 0u, 1.0 / (double)c.R, dot);
+    }
+
     // precond_camnew (LOBPCG.cpp:168-199): w <- IFFT(FFT(w)/N^3 / (G+|Tau|+0.02))
     // minus its mean (= the same with the DC set to zero).
-    void precond(Ctx &c, void *w) override
+    template <class OpIn> void precond_in(Ctx &c, const void *in, OpIn opin, void *w)
     {
         if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
-                sj_round_trip(c, w, OpNone{}, nullptr, [] {}, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab},
+                sj_round_trip(c, in, opin, nullptr, [] {}, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab},
                               OpStore<T>{(T *)w}, nullptr);
                 return;
             }
         }
-        rows<false, true>(c, w, c.W, OpNone{}, nullptr);
+        rows<false, true>(c, in, c.W, opin, nullptr);
         cols_j<true, false, true>(c, c.W, c.W, SOpNone{}, nullptr);
         cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab}, nullptr);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
         rows<true, false>(c, c.W, nullptr, OpStore<T>{(T *)w}, nullptr);
+    }
+    void precond(Ctx &c, void *w) override { precond_in(c, w, OpNone{}, w); }
+    void precond_resid(Ctx &c, const void *u, const void *v, double alpha, void *w) override
+    {
+        precond_in(c, u, OpResid<T>{alpha, (const T *)v}, w);
     }
 
     void gconv(Ctx &c, const void *in, void *out) override

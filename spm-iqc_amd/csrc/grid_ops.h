@@ -275,19 +275,37 @@ template <typename T> struct OpHvIn {
 
 // hv_cam epilogue (LOBPCG.cpp:136-156): hv = x + c(phi) v - mean(c v), with x
 // = ifft(G v_hat, DC = 0) (the DC of G v_hat is the mean of ifft(G v_hat)).
-template <typename T> struct OpHvOut {
-    static constexpr int NRED = 0;
+// DOT: also the sum of v hv (the stored hv), for <v, H v> (LOBPCG.cpp:281-283).
+template <typename T, bool DOT = false> struct OpHvOut {
+    static constexpr int NRED = DOT ? 1 : 0;
     static constexpr unsigned MAXM = 0;
     double Tau, Gamma;
     const T *phi, *v;
     const double *mean_cv;
     T *hv;
-    __device__ __forceinline__ T operator()(T x, long g, double *) const
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const
     {
-        const double u = (double)phi[g];
-        hv[g] = (T)((double)x + (-Tau - 2 * Gamma * u + 3 * u * u) * (double)v[g] - *mean_cv);
+        const double u = (double)phi[g], vg = (double)v[g];
+        const T y = (T)((double)x + (-Tau - 2 * Gamma * u + 3 * u * u) * vg - *mean_cv);
+        hv[g] = y;
+        if constexpr (DOT) acc[0] += vg * (double)y;
         return x;
     }
+};
+
+// The residual u - alpha v of one LOBPCG block member (LOBPCG.cpp:284), formed
+// as the input of the preconditioner's first pass instead of in a field of its own.
+template <typename T> struct OpResid {
+    static constexpr int NRED = 0;
+    static constexpr unsigned MAXM = 0;
+    double alpha;
+    const T *v;
+    __device__ __forceinline__ const T *aux() const { return v; }
+    __device__ __forceinline__ T operator()(T x, T vg, long, double *) const
+    {
+        return (T)((double)x - alpha * (double)vg);
+    }
+    __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, v[g], g, acc); }
 };
 
 // Natural_Forces / MinMax_Forces input (SPM.cpp:53-86 with beta = 1): the

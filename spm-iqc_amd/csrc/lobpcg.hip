@@ -155,30 +155,359 @@ static int lob_block(spm_ctx *h, void *Vout, int k, size_t extra_fields, LobBloc
     return SPM_OK;
 }
 
-// One iteration of hiosd_lobpcg_inip (LOBPCG.cpp:279-449) at phi; ritz (k) may be null.
+// ---------------------------------------------------------------- block sweep
+// Small dense matrices, row-major.
+using Mat = std::vector<double>;
+// C (r x c) = A^T (A: n x r) B (n x c)
+static Mat mtm(const Mat &A, const Mat &B, int n, int r, int c)
+{
+    Mat C((size_t)r * c, 0.0);
+    for (int l = 0; l < n; ++l)
+        for (int i = 0; i < r; ++i) {
+            const double a = A[(size_t)l * r + i];
+            if (a != 0.0)
+                for (int j = 0; j < c; ++j) C[(size_t)i * c + j] += a * B[(size_t)l * c + j];
+        }
+    return C;
+}
+// C (r x c) = A (r x n) B (n x c)
+static Mat mm(const Mat &A, const Mat &B, int r, int n, int c)
+{
+    Mat C((size_t)r * c, 0.0);
+    for (int i = 0; i < r; ++i)
+        for (int l = 0; l < n; ++l) {
+            const double a = A[(size_t)i * n + l];
+            if (a != 0.0)
+                for (int j = 0; j < c; ++j) C[(size_t)i * c + j] += a * B[(size_t)l * c + j];
+        }
+    return C;
+}
+// the (i0.., j0..) block (r x c) of the n-column matrix G
+static Mat sub(const Mat &G, int n, int i0, int j0, int r, int c)
+{
+    Mat S((size_t)r * c);
+    for (int i = 0; i < r; ++i)
+        for (int j = 0; j < c; ++j) S[(size_t)i * c + j] = G[(size_t)(i0 + i) * n + j0 + j];
+    return S;
+}
+
+// Cholesky with deflation of an orthogonal complement's Gram S = G - P^T P
+// formed from Grams (G: the fields' own, P: their components along an
+// orthonormal basis), where the subtraction loses digits.  The keep/drop
+// decisions and R agree with those of the explicitly projected fields only
+// where each remaining squared norm d_i is far above the rounding of the
+// subtraction, about 1e-13 of the field's own squared norm g_i: d_i >= 1e-6
+// g_i for a kept field, d_i + 1e3 (1e-13 g_i) < tol^2 for a dropped one.
+static bool chol_from_grams(int n, const Mat &S, const double *g, double tol, bool *keep, Mat &Rinv)
+{
+    std::vector<double> piv(n);
+    Rinv.assign((size_t)n * n, 0.0);
+    chol_inv(n, S.data(), tol, true, keep, Rinv.data(), piv.data());
+    for (int i = 0; i < n; ++i) {
+        const double noise = 1e-13 * g[i];
+        if (keep[i] ? !(piv[i] >= 1e-6 * g[i]) : !(piv[i] + 1e3 * noise < tol * tol)) return false;
+    }
+    return true;
+}
+
+// Rayleigh-Ritz of the symmetric pencil (A, B), B the measured Gram of the
+// basis: B = R^T R, the eigenpairs (w, y) of R^-T A R^-1, Z = R^-1 y (B-orthonormal
+// columns, ascending w).  Returns false when B is not positive definite.
+static bool ritz_pencil(int m, const Mat &A, const Mat &B, std::vector<double> &w, Mat &Z)
+{
+    Mat Rinv;
+    std::vector<char> keepb(m);
+    Rinv.assign((size_t)m * m, 0.0);
+    std::vector<double> piv(m);
+    chol_inv(m, B.data(), -1.0, false, (bool *)keepb.data(), Rinv.data(), piv.data());
+    for (int i = 0; i < m; ++i)
+        if (!keepb[i] || !(piv[i] > 1e-8 * B[(size_t)i * m + i])) return false;
+    Mat C = mm(mtm(Rinv, A, m, m, m), Rinv, m, m, m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < i; ++j) C[(size_t)i * m + j] = C[(size_t)j * m + i] = 0.5 * (C[(size_t)i * m + j] + C[(size_t)j * m + i]);
+    Mat Y;
+    sym_eig(m, C, w, Y);
+    Z = mm(Rinv, Y, m, m, m);
+    return true;
+}
+
+static int lobpcg_sweep_ref(spm_ctx *h, const void *phi, LobBlock &b, double *ritz, bool from_w = false);
+static int lobpcg_rayleigh_ritz(spm_ctx *h, const void *phi, LobBlock &b, double *ritz);
+
+// One iteration of hiosd_lobpcg_inip (LOBPCG.cpp:279-449) at phi, without fixed
+// fields V0, in the fewest passes over the block:
+//   U = H V and alpha_i = <V_i, U_i> from the same passes (hv_dot);
+//   W = precond(U - alpha V), the residual formed in the preconditioner's input pass;
+//   one Gram of [V, W, Vp]; from it the orthonormalisation of W against V and of
+//   Vp against [V, W] (in order, with the reference's deflation at norm 1e-8) as
+//   coefficient matrices, applied in one combine that writes both blocks.  Where
+//   the complement's Gram cancels too much to decide (Vp close to span V, which
+//   is where the iteration converges) Vp is projected explicitly and its Gram
+//   measured again; where W's does, the sweep takes the reference-order path;
+//   Y = H W, Up = H Vp on the kept fields;
+//   one pass over [X | YY], X = [V, W, Vp] (kept), YY = [U, Y, Up]: A = X^T YY,
+//   B = X^T X and the means of X; the Ritz pencil (A, B); the lowest k Ritz
+//   vectors, their means removed and orthonormalised in order, in one combine
+//   whose coefficients come from B and the means.
+// Each step is the reference's in exact arithmetic; the Ritz values use the
+// measured B rather than assuming the basis orthonormal.
 static int lobpcg_sweep(spm_ctx *h, const void *phi, LobBlock &b, double *ritz)
+{
+    if (!b.V0.empty()) return lobpcg_sweep_ref(h, phi, b, ritz);
+    Ctx &c = h->c;
+    const int k = (int)b.V.size();
+    std::vector<void *> &V = b.V, &U = b.U, &W = b.W, &Y = b.Y, &Vp = b.Vp, &Up = b.Up, &T = b.T;
+    bool *keepW = b.keepW(), *keepP = b.keepP();
+    // U = H V, alpha (:281-284)
+    double *adev = block_small(h);
+    if (!adev) return spm_fail(SPM_ENOMEM, "lobpcg: block scratch");
+    for (int i = 0; i < k; ++i) c.g->hv_dot(c, phi, V[i], U[i], adev + i);
+    RCHK(spm_launch_status("lobpcg hv"));
+    std::vector<double> alpha(k);
+    HIPCHK(hipMemcpyAsync(h->h_pin, adev, sizeof(double) * k, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    std::memcpy(alpha.data(), h->h_pin, sizeof(double) * k);
+    // W = precond(U - alpha V) (:284-300)
+    for (int i = 0; i < k; ++i) c.g->precond_resid(c, U[i], V[i], alpha[i], W[i]);
+    RCHK(spm_launch_status("lobpcg precond"));
+    // G = [V, W, Vp]^T [V, W, Vp]
+    const int n3 = 3 * k;
+    std::vector<const void *> X3(n3);
+    for (int i = 0; i < k; ++i) {
+        X3[i] = V[i];
+        X3[k + i] = W[i];
+        X3[2 * k + i] = Vp[i];
+    }
+    Mat G((size_t)n3 * n3);
+    RCHK(block_gram(h, X3.data(), n3, X3.data(), n3, G.data()));
+    // W against V, then in order (:301-318): W' = W - V PW, Gram WW - PW^T PW
+    const Mat PW = sub(G, n3, 0, k, k, k), Gww = sub(G, n3, k, k, k, k);
+    Mat SW = Gww, RW;
+    {
+        const Mat PtP = mtm(PW, PW, k, k, k);
+        for (size_t e = 0; e < SW.size(); ++e) SW[e] -= PtP[e];
+    }
+    std::vector<double> gw(k), gp(k);
+    for (int i = 0; i < k; ++i) {
+        gw[i] = Gww[(size_t)i * k + i];
+        gp[i] = G[(size_t)(2 * k + i) * n3 + 2 * k + i];
+    }
+    if (!chol_from_grams(k, SW, gw.data(), 1e-8, keepW, RW)) return lobpcg_sweep_ref(h, phi, b, ritz, true);
+    // W^ = [V, W] CW, CW = [-PW RW^-1; RW^-1]
+    Mat CW((size_t)2 * k * k, 0.0);
+    {
+        const Mat PR = mm(PW, RW, k, k, k);
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j) {
+                CW[(size_t)i * k + j] = -PR[(size_t)i * k + j];
+                CW[(size_t)(k + i) * k + j] = RW[(size_t)i * k + j];
+            }
+    }
+    // Vp against V and W^, then in order (:319-331): PV = V^T Vp, PH = W^^T Vp = CW^T [V, W]^T Vp
+    const Mat PV = sub(G, n3, 0, 2 * k, k, k);
+    const Mat PH = mtm(CW, sub(G, n3, 0, 2 * k, 2 * k, k), 2 * k, k, k);
+    Mat SP = sub(G, n3, 2 * k, 2 * k, k, k), RP;
+    {
+        const Mat A1 = mtm(PV, PV, k, k, k), A2 = mtm(PH, PH, k, k, k);
+        for (size_t e = 0; e < SP.size(); ++e) SP[e] -= A1[e] + A2[e];
+    }
+    const bool fastP = chol_from_grams(k, SP, gp.data(), 1e-8, keepP, RP);
+    // Vp' = Vp - V PV - W^ PH = [V, W, Vp] D, D = [-PV - CW_V PH; -CW_W PH; I]
+    Mat D((size_t)n3 * k, 0.0);
+    {
+        const Mat CH = mm(CW, PH, 2 * k, k, k);
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j) {
+                D[(size_t)i * k + j] = -PV[(size_t)i * k + j] - CH[(size_t)i * k + j];
+                D[(size_t)(k + i) * k + j] = -CH[(size_t)(k + i) * k + j];
+                D[(size_t)(2 * k + i) * k + j] = i == j ? 1.0 : 0.0;
+            }
+    }
+    const Mat DP = fastP ? mm(D, RP, n3, k, k) : D;
+    {   // one combine: [V, W, Vp] -> [W^, Vp^ (or Vp')]
+        Mat coef((size_t)n3 * 2 * k, 0.0);
+        for (int a = 0; a < n3; ++a)
+            for (int j = 0; j < k; ++j) {
+                if (a < 2 * k) coef[(size_t)a * 2 * k + j] = CW[(size_t)a * k + j];
+                coef[(size_t)a * 2 * k + k + j] = DP[(size_t)a * k + j];
+            }
+        std::vector<void *> out(2 * k);
+        for (int i = 0; i < k; ++i) {
+            out[i] = W[i];
+            out[k + i] = Vp[i];
+        }
+        RCHK(block_combine(h, X3.data(), n3, out.data(), 2 * k, coef.data(), nullptr));
+    }
+    if (!fastP) {
+        // Vp' measured against [V, W^ (kept)]: c = M^-1 [V, W^]^T Vp', Gram Vp'^T Vp' - b^T c
+        Mat G2((size_t)n3 * n3);
+        RCHK(block_gram(h, X3.data(), n3, X3.data(), n3, G2.data()));
+        std::vector<int> bi; // basis rows of G2: V and the kept W^
+        for (int i = 0; i < k; ++i) bi.push_back(i);
+        for (int i = 0; i < k; ++i)
+            if (keepW[i]) bi.push_back(k + i);
+        const int nb = (int)bi.size();
+        Mat M((size_t)nb * nb), Bv((size_t)nb * k);
+        for (int r = 0; r < nb; ++r) {
+            for (int q = 0; q < nb; ++q) M[(size_t)r * nb + q] = G2[(size_t)bi[r] * n3 + bi[q]];
+            for (int j = 0; j < k; ++j) Bv[(size_t)r * k + j] = G2[(size_t)bi[r] * n3 + 2 * k + j];
+        }
+        Mat MR((size_t)nb * nb);
+        bool orth = true;
+        {
+            std::vector<char> kb(nb);
+            std::vector<double> piv(nb);
+            chol_inv(nb, M.data(), -1.0, false, (bool *)kb.data(), MR.data(), piv.data());
+            for (int r = 0; r < nb; ++r) orth = orth && kb[r] && piv[r] > 0.5 * M[(size_t)r * nb + r];
+        }
+        if (!orth) { // [V, W^] far from orthonormal: Vp' against it in the reference's order
+            std::vector<const void *> Bf;
+            for (int r = 0; r < nb; ++r) Bf.push_back(X3[bi[r]]);
+            RCHK(block_ortho(h, Vp.data(), k, Bf.data(), nb, 1e-8, keepP));
+            return lobpcg_rayleigh_ritz(h, phi, b, ritz);
+        }
+        const Mat Cc = mm(MR, mtm(MR, Bv, nb, nb, k), nb, nb, k); // M^-1 Bv
+        Mat S2 = sub(G2, n3, 2 * k, 2 * k, k, k);
+        {
+            const Mat BtC = mtm(Bv, Cc, nb, k, k);
+            for (size_t e = 0; e < S2.size(); ++e) S2[e] -= BtC[e];
+        }
+        Mat R2((size_t)k * k);
+        chol_inv(k, S2.data(), 1e-8, true, keepP, R2.data(), nullptr);
+        // Vp^ = (Vp' - [V, W^] c) R2^-1
+        std::vector<const void *> in;
+        for (int r = 0; r < nb; ++r) in.push_back(X3[bi[r]]);
+        for (int i = 0; i < k; ++i) in.push_back(Vp[i]);
+        const int nin = nb + k;
+        Mat coef((size_t)nin * k, 0.0);
+        const Mat CR = mm(Cc, R2, nb, k, k);
+        for (int r = 0; r < nb; ++r)
+            for (int j = 0; j < k; ++j) coef[(size_t)r * k + j] = -CR[(size_t)r * k + j];
+        for (int i = 0; i < k; ++i)
+            for (int j = 0; j < k; ++j) coef[(size_t)(nb + i) * k + j] = R2[(size_t)i * k + j];
+        RCHK(block_combine(h, in.data(), nin, Vp.data(), k, coef.data(), nullptr));
+    }
+    return lobpcg_rayleigh_ritz(h, phi, b, ritz);
+}
+
+// Y = H W, Up = H Vp on the kept fields, then the Rayleigh-Ritz step, the mean
+// removal and the orthonormalisation of the new V (LOBPCG.cpp:332-449) from one
+// pass over [X | YY] and one combine.
+static int lobpcg_rayleigh_ritz(spm_ctx *h, const void *phi, LobBlock &b, double *ritz)
 {
     Ctx &c = h->c;
     const int k = (int)b.V.size();
     std::vector<void *> &V = b.V, &U = b.U, &W = b.W, &Y = b.Y, &Vp = b.Vp, &Up = b.Up, &T = b.T;
     bool *keepV = b.keepV(), *keepW = b.keepW(), *keepP = b.keepP();
-    for (int i = 0; i < k; ++i) c.g->hv(c, phi, V[i], U[i]);
+    for (int i = 0; i < k; ++i)
+        if (keepW[i]) c.g->hv(c, phi, W[i], Y[i]);
+    for (int i = 0; i < k; ++i)
+        if (keepP[i]) c.g->hv(c, phi, Vp[i], Up[i]);
     RCHK(spm_launch_status("lobpcg hv"));
-    std::vector<double> VU(k * k);
-    RCHK(block_gram(h, cv(V).data(), k, cv(U).data(), k, VU.data()));
-    {   // W = U - alpha V  (FuncsLinear2Cplx, :284)
-        std::vector<const void *> in(2 * k);
-        std::vector<double> coef(2 * k * k, 0.0);
-        for (int i = 0; i < k; ++i) {
-            in[i] = U[i];
-            in[k + i] = V[i];
-            coef[i * k + i] = 1.0;
-            coef[(k + i) * k + i] = -VU[i * k + i];
-        }
-        RCHK(block_combine(h, in.data(), 2 * k, W.data(), k, coef.data(), nullptr));
+    std::vector<const void *> X, YY;
+    for (int i = 0; i < k; ++i) {
+        X.push_back(V[i]);
+        YY.push_back(U[i]);
     }
-    for (int i = 0; i < k; ++i) c.g->precond(c, W[i]);
-    RCHK(spm_launch_status("lobpcg precond"));
+    for (int i = 0; i < k; ++i)
+        if (keepW[i]) {
+            X.push_back(W[i]);
+            YY.push_back(Y[i]);
+        }
+    for (int i = 0; i < k; ++i)
+        if (keepP[i]) {
+            X.push_back(Vp[i]);
+            YY.push_back(Up[i]);
+        }
+    const int m = (int)X.size();
+    Mat A((size_t)m * m), B((size_t)m * m), Z;
+    std::vector<double> mu(m), w;
+    RCHK(block_gram_xy(h, X.data(), m, YY.data(), m, A.data(), B.data(), mu.data()));
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < i; ++j) {
+            A[(size_t)i * m + j] = A[(size_t)j * m + i] = 0.5 * (A[(size_t)i * m + j] + A[(size_t)j * m + i]);
+            B[(size_t)i * m + j] = B[(size_t)j * m + i] = 0.5 * (B[(size_t)i * m + j] + B[(size_t)j * m + i]);
+        }
+    if (!ritz_pencil(m, A, B, w, Z)) { // B singular: the reference's matrix, the basis taken as orthonormal
+        Mat I((size_t)m * m, 0.0);
+        for (int i = 0; i < m; ++i) I[(size_t)i * m + i] = 1.0;
+        B = I;
+        Mat Ps = A;
+        sym_eig(m, Ps, w, Z);
+    }
+    // T = X Zk - mean, orthonormalised in order: Gram Zk^T B Zk - nu nu^T, nu = Zk^T mu
+    Mat Zk((size_t)m * k);
+    for (int a = 0; a < m; ++a)
+        for (int i = 0; i < k; ++i) Zk[(size_t)a * k + i] = Z[(size_t)a * m + i];
+    std::vector<double> nu(k, 0.0);
+    for (int i = 0; i < k; ++i)
+        for (int a = 0; a < m; ++a) nu[i] += Zk[(size_t)a * k + i] * mu[a];
+    Mat ST = mtm(Zk, mm(B, Zk, m, m, k), m, k, k);
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) ST[(size_t)i * k + j] -= nu[i] * nu[j];
+    Mat RT((size_t)k * k);
+    std::vector<double> piv(k);
+    chol_inv(k, ST.data(), -1.0, false, keepV, RT.data(), piv.data());
+    bool fastT = true;
+    for (int i = 0; i < k; ++i) fastT = fastT && keepV[i] && piv[i] >= 1e-6;
+    if (fastT) {
+        const Mat CT = mm(Zk, RT, m, k, k);
+        std::vector<double> shift(k, 0.0);
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < k; ++i) shift[j] -= nu[i] * RT[(size_t)i * k + j];
+        RCHK(block_combine(h, X.data(), m, T.data(), k, CT.data(), shift.data()));
+    } else { // a Ritz vector close to a constant: mean removal and orthonormalisation in turn
+        RCHK(block_combine(h, X.data(), m, T.data(), k, Zk.data(), nullptr));
+        std::vector<double> mean(k), eye(k * k, 0.0), shift(k);
+        RCHK(block_means(h, cv(T).data(), k, mean.data()));
+        for (int i = 0; i < k; ++i) {
+            eye[i * k + i] = 1.0;
+            shift[i] = -mean[i];
+        }
+        RCHK(block_combine(h, cv(T).data(), k, T.data(), k, eye.data(), shift.data()));
+        RCHK(block_ortho(h, T.data(), k, nullptr, 0, -1.0, keepV));
+    }
+    // Vp <- V, Up <- U (:391-400); V <- T
+    std::vector<void *> oldVp = Vp, oldUp = Up;
+    Vp = V;
+    Up = U;
+    V = T;
+    T = oldVp;
+    U = oldUp;
+    if (ritz)
+        for (int i = 0; i < k; ++i) ritz[i] = w[i];
+    return SPM_OK;
+}
+
+// The sweep in the reference's order of operations (two projection rounds and
+// CholQR2 per block, the Rayleigh-Ritz matrix assuming an orthonormal basis):
+// with fixed fields V0 (GOSD), or from_w: W's complement Gram did not decide,
+// U, alpha and W = precond(U - alpha V) already formed.
+static int lobpcg_sweep_ref(spm_ctx *h, const void *phi, LobBlock &b, double *ritz, bool from_w)
+{
+    Ctx &c = h->c;
+    const int k = (int)b.V.size();
+    std::vector<void *> &V = b.V, &U = b.U, &W = b.W, &Y = b.Y, &Vp = b.Vp, &Up = b.Up, &T = b.T;
+    bool *keepV = b.keepV(), *keepW = b.keepW(), *keepP = b.keepP();
+    if (!from_w) {
+        for (int i = 0; i < k; ++i) c.g->hv(c, phi, V[i], U[i]);
+        RCHK(spm_launch_status("lobpcg hv"));
+        std::vector<double> VU(k * k);
+        RCHK(block_gram(h, cv(V).data(), k, cv(U).data(), k, VU.data()));
+        {   // W = U - alpha V  (FuncsLinear2Cplx, :284)
+            std::vector<const void *> in(2 * k);
+            std::vector<double> coef(2 * k * k, 0.0);
+            for (int i = 0; i < k; ++i) {
+                in[i] = U[i];
+                in[k + i] = V[i];
+                coef[i * k + i] = 1.0;
+                coef[(k + i) * k + i] = -VU[i * k + i];
+            }
+            RCHK(block_combine(h, in.data(), 2 * k, W.data(), k, coef.data(), nullptr));
+        }
+        for (int i = 0; i < k; ++i) c.g->precond(c, W[i]);
+        RCHK(spm_launch_status("lobpcg precond"));
+    }
     const int k0 = (int)b.V0.size();
     if (k0 == 0) {
         RCHK(block_ortho(h, W.data(), k, cv(V).data(), k, 1e-8, keepW));

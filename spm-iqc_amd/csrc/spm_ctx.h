@@ -86,7 +86,11 @@ struct GridIface {
     virtual void energy(Ctx &c, const void *phi) = 0;
     virtual void ngrad(Ctx &c, const void *phi, void *grad) = 0; // scal[SC_GN2] = nrmp^2
     virtual void hv(Ctx &c, const void *phi, const void *v, void *hv) = 0;
+    // hv, and *dot (device) = inpfp(v, hv)
+    virtual void hv_dot(Ctx &c, const void *phi, const void *v, void *hv, double *dot) = 0;
     virtual void precond(Ctx &c, void *w) = 0;
+    // w = precond(u - alpha v)  (w may not alias u or v)
+    virtual void precond_resid(Ctx &c, const void *u, const void *v, double alpha, void *w) = 0;
     // out = IFFT(G FFT(in) / N^3) with the DC set to zero (the non-local part of ngrad_cam)
     virtual void gconv(Ctx &c, const void *in, void *out) = 0;
     // x <- IFFT(FFT(x + nln(x) - coef * (*s) * v) / N^3 / (1 + G)), DC = 0
