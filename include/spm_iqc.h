@@ -58,7 +58,22 @@ typedef struct {
                        128 and 256, 4 otherwise), 4 = one pass per axis (any
                        N), 2 = the j transform split between the row and i
                        passes (N = 64, 128 and 256 only) */
+    int step_variant; /* variant of the two-pass gradient-flow step: 0 = the
+                       built-in choice for N and precision, -1 = the tuned one
+                       (spm_tune_step; measured on first use and kept in the
+                       tuning cache), or SPM_STEP_EXPLICIT | SPM_STEP_* flags.
+                       Every variant computes the same values. */
 } spm_params;
+
+/* Two-pass step variants (spm_params.step_variant). */
+#define SPM_STEP_EXPLICIT 8  /* the flags below are the variant */
+#define SPM_STEP_GATHER_U 1  /* pass 2 gathers the state spectrum where it is used
+                                (otherwise: loaded at the start of the pass) */
+#define SPM_STEP_U_CACHED 2  /* state loads and stores without non-temporal hints */
+#define SPM_STEP_U_STREAM 4  /* ... with them (neither: with them when the state and
+                                the work spectrum exceed 200 MB together) */
+#define SPM_STEP_U_MASK 6
+#define SPM_STEP_VARIANTS 6  /* flag sets 0..5 (U_CACHED | U_STREAM is not one) */
 
 typedef struct {
     int N, nh, nhp, precision;
@@ -268,6 +283,13 @@ int spm_shapes(int N, int *count, int *tpp, int max_shapes);
  * $HOME/.cache/spm-iqc/shapes.txt), which spm_create and spm_batch_create
  * read for shape -1. */
 int spm_tune(const spm_params *p, int nmem, int steps, int *best, double *us_per_step);
+/* The same for the variants of the two-pass step (p->N = 64, 128 or 256):
+ * every flag set 0 .. SPM_STEP_VARIANTS-1 timed on `steps` graph-replayed
+ * iterations of a batch of nmem members at p->shape; *best <- the fastest flag
+ * set (without SPM_STEP_EXPLICIT); us_per_variant (may be NULL,
+ * SPM_STEP_VARIANTS doubles) <- the times.  Recorded in the tuning cache,
+ * which spm_create and spm_batch_create read for step_variant -1. */
+int spm_tune_step(const spm_params *p, int nmem, int steps, int *best, double *us_per_variant);
 
 #ifdef __cplusplus
 }

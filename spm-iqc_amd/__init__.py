@@ -37,7 +37,8 @@ class SpmError(RuntimeError):
 class _Params(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("precision", ctypes.c_int), ("L", ctypes.c_double),
                 ("kscale", ctypes.c_double), ("Tau", ctypes.c_double), ("Gamma", ctypes.c_double),
-                ("dt", ctypes.c_double), ("shape", ctypes.c_int), ("step_passes", ctypes.c_int)]
+                ("dt", ctypes.c_double), ("shape", ctypes.c_int), ("step_passes", ctypes.c_int),
+                ("step_variant", ctypes.c_int)]
 
 
 class _Layout(ctypes.Structure):
@@ -100,6 +101,7 @@ SIGNATURES = {
     "spm_batch_release": (_i, [_c, _i]),
     "spm_shapes": (_i, [_i, _pi, _c, _i]),
     "spm_tune": (_i, [ctypes.POINTER(_Params), _i, _i, _pi, _c]),
+    "spm_tune_step": (_i, [ctypes.POINTER(_Params), _i, _i, _pi, _c]),
 }
 
 
@@ -141,9 +143,9 @@ def _torch():
     return torch
 
 
-def _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes=0):
+def _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes=0, step_variant=0):
     return _Params(int(N), int(precision), float(L), float(kscale), float(Tau), float(Gamma), float(dt), int(shape),
-                   int(step_passes))
+                   int(step_passes), int(step_variant))
 
 
 def shapes(N):
@@ -167,6 +169,23 @@ def tune(N, precision=FP64, nmem=1, steps=20, L=26.0, kscale=1.0, Tau=-0.0061, G
     return best.value, us.tolist()
 
 
+# Two-pass step variants (spm_params.step_variant): 0 = built-in, -1 = tuned,
+# else STEP_EXPLICIT | flags
+STEP_EXPLICIT, STEP_GATHER_U, STEP_U_CACHED, STEP_U_STREAM, STEP_VARIANTS = 8, 1, 2, 4, 6
+
+
+def tune_step(N, precision=FP64, nmem=1, steps=20, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3):
+    """Time every two-pass step variant (flag sets 0 .. STEP_VARIANTS-1) of N;
+    returns (best flags, us_per_step list).  Recorded for step_variant=-1."""
+    _torch()
+    p = _params(N, precision, L, kscale, Tau, Gamma, dt, 0, 2)
+    best = ctypes.c_int()
+    us = np.zeros(STEP_VARIANTS)
+    _check(lib().spm_tune_step(ctypes.byref(p), int(nmem), int(steps), ctypes.byref(best), us.ctypes.data_as(_c)),
+           "spm_tune_step")
+    return best.value, us.tolist()
+
+
 class Grid:
     """One device context: an N^3 grid, the model parameters and a precision.
 
@@ -176,10 +195,10 @@ class Grid:
     """
 
     def __init__(self, N, precision=FP64, L=26.0, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3, shape=0,
-                 step_passes=0):
+                 step_passes=0, step_variant=0):
         torch = _torch()
         self.torch = torch
-        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes)
+        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes, step_variant)
         h = _c()
         _check(lib().spm_create(ctypes.byref(p), ctypes.byref(h)), "spm_create")
         self.h = h
@@ -454,10 +473,10 @@ class Batch:
     torch tensors on the GPU in the batch precision."""
 
     def __init__(self, N, precision=FP64, capacity=8, hist_cap=4096, L=26.0, kscale=1.0, Tau=-0.0061,
-                 Gamma=0.30, dt=0.3, shape=0, step_passes=0):
+                 Gamma=0.30, dt=0.3, shape=0, step_passes=0, step_variant=0):
         torch = _torch()
         self.torch = torch
-        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes)
+        p = _params(N, precision, L, kscale, Tau, Gamma, dt, shape, step_passes, step_variant)
         h = _c()
         _check(lib().spm_batch_create(ctypes.byref(p), int(capacity), int(hist_cap), ctypes.byref(h)),
                "spm_batch_create")

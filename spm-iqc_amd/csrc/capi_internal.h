@@ -48,6 +48,8 @@ void spm_ctx_free(spm_ctx *h);
 void spm_ctl_init(spm_ctx *h, int m, int max_iter, double res_tol, double gn_tol);
 // shape recorded by the tuner for (p->N, p->precision, nmem), tuning on a miss
 int spm_resolve_shape(const spm_params *p, int nmem, int *shape);
+// two-pass step variant recorded by the tuner for (p->N, p->precision, nmem), tuning on a miss
+int spm_resolve_step(const spm_params *p, int nmem, int *variant);
 int spm_dev_alloc(spm_ctx *h, void **p, size_t bytes);
 int spm_launch_status(const char *what);
 int spm_read_scalars(spm_ctx *h, const double *dev, int n, double *out);

@@ -414,13 +414,24 @@ struct GridImpl final : GridIface {
         const int kf = (int)std::min<long>(64, c.nb_j);
         const int nm = c.nmem;
         const long bs = c.C;
-        const bool nt = 2.0 * (double)bs * sizeof(CT) * nm > 200e6; // W + U beyond the 256 MB last-level cache
+        // U with non-temporal hints when W + U overflow the 256 MB last-level cache,
+        // unless the step variant fixes the hint
+        const int um = (c.sjv & SPM_STEP_U_MASK) >> 1;
+        const bool nt = um == 0 ? 2.0 * (double)bs * sizeof(CT) * nm > 200e6 : um == 2;
         if constexpr (N == 64 || N == 128 || N == 256) {
             if (c.gf2) {
                 using S = SJT;
-                sj_cols<true, true>(c, c.W, c.W,
-                                    SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
-                                    c.slot(3), c.ctl, ProSJDensity{c.slot(2), S::R_BLOCKS, c.slot(4)}, nm, bs);
+                const ProSJDensity pd{c.slot(2), S::R_BLOCKS, c.slot(4)};
+                if (c.sjv & SPM_STEP_GATHER_U)
+                    sj_cols<true, true>(
+                        c, c.W, c.W,
+                        SOpGradFlowStep<T, N, false>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
+                        c.slot(3), c.ctl, pd, nm, bs);
+                else
+                    sj_cols<true, true>(
+                        c, c.W, c.W,
+                        SOpGradFlowStep<T, N, true>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
+                        c.slot(3), c.ctl, pd, nm, bs);
                 sj_rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr, c.prm, 0}, c.slot(2), c.ctl,
                                     ProSJFinalize{c.slot(3), S::C_BLOCKS, c.slot(4), (double)c.R, c.ctl, c.hist,
                                                   3 * c.hist_cap},

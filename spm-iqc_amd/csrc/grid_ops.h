@@ -469,13 +469,14 @@ template <typename T, int N> struct SOpEnergy {
 // Sums, for the pre-step state: w G |U|^2 (2 E1), w |G U - f_hat|^2 over
 // k != 0 (||grad F||^2 by Parseval), and max |Re(U' - U)| (the reference's
 // residual).  v becomes U' (then inverse-transformed by the same pass).
-// U is loaded at the start of the pass (PRELOAD) so its latency overlaps the
-// forward transform.  G and D = 1 / (1 + dt G) come from one table entry
-// (one gather, and a multiply instead of two FP64 divides per entry).
-template <typename T, int N> struct SOpGradFlowStep {
+// PRE: U is loaded at the start of the pass (PRELOAD) so its latency overlaps
+// the forward transform; otherwise it is gathered where it is used (fewer
+// registers held across the transform).  G and D = 1 / (1 + dt G) come from one
+// table entry (one gather, and a multiply instead of two FP64 divides per entry).
+template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
     static constexpr int NRED = 3;
     static constexpr unsigned MAXM = 4u; // acc[2] is a max
-    static constexpr bool PRELOAD = true, FINALIZE = false;
+    static constexpr bool PRELOAD = PRE, FINALIZE = false;
     double sc, dt;
     const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
@@ -516,6 +517,7 @@ template <typename T, int N> struct SOpGradFlowStep {
         const int a = kfold(i, N), b = kfold(j, N);
         const int n2 = a * a + b * b + kz * kz;
         const C2<T> gd = GD[n2]; // gathered here: fetching all E entries up front costs occupancy
+        if constexpr (!PRELOAD) u = load_pre(s);
         if constexpr (sizeof(T) == 4) { // FP32 fields: the update in FP32, the three sums in FP64
             const float g = gd.x, d = gd.y, w = (kz == 0 || kz == N / 2) ? 1.0f : 2.0f;
             const float fr = v.x * (float)sc, fi = v.y * (float)sc, ur = u.x, ui = u.y, dtf = (float)dt;

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "grid_ops.h"
+#include "spm_iqc.h"
 
 namespace spm {
 
@@ -54,6 +55,9 @@ struct Ctx {
     int nmem = 1;
     double *prm = nullptr;
     int shape = 0; // FFT pass shape (index into grid_shapes(N))
+    // two-pass step variant (SPM_STEP_* flags without SPM_STEP_EXPLICIT): bit 0
+    // gathers U in pass 2 where it is used, bits 1-2 the cache hint of U
+    int sjv = 0;
     int gf2 = 0;   // N = 256: gradient-flow step in two passes (U and W in the two-pass layout)
 
     double *slot(int s) { return part + (long)s * part_slot; }

@@ -1,14 +1,17 @@
-// tune.hip -- choice of the FFT pass shapes (threads per pencil of the row,
-// j and i passes) by measurement, with a cache of the results on disk: the
+// tune.hip -- choice by measurement of the FFT pass shapes (threads per pencil
+// of the row, j and i passes of the four-pass step and the operators) and of
+// the two-pass step's variant, with a cache of the results on disk: the
 // counterpart of the reference's FFTW_PATIENT planning and wisdom files
 // (My_mpi_fft_set_up, My_FFT.cpp:45-97).
 //
-// Cache: one line per measurement, "arch N precision members shape us_per_step
+// Cache: one line per measurement, "arch N precision members choice us_per_step
 // table", appended to $SPM_TUNE_CACHE (default $HOME/.cache/spm-iqc/shapes.txt);
 // the last line for a key wins.  Members are bucketed to the next power of two.
-// "table" is a signature of N's shape table in this build: a line recorded with
-// another table (a rebuild that added, dropped or reordered shapes) never
-// matches, and the shape is measured again.
+// "table" is a signature of what was chosen from: of N's shape table in this
+// build for shapes (a line recorded with another table -- a rebuild that added,
+// dropped or reordered shapes -- never matches, and the shape is measured
+// again); "sj" and the variant count for two-pass step variants, whose lines
+// carry the arch as "<arch>/sj".
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
@@ -92,9 +95,11 @@ void load_cache() // g_mu held
     std::fclose(f);
 }
 
-void record(const std::string &ar, int N, int prec, int nmem, int shape, double us) // g_mu held
+std::string step_sig() { return "sj" + std::to_string(SPM_STEP_VARIANTS); }
+std::string step_arch() { return arch() + "/sj"; }
+
+void record(const std::string &ar, int N, int prec, int nmem, int shape, double us, const std::string &sig) // g_mu held
 {
-    const std::string sig = table_sig(N);
     g_table[key(ar, N, prec, nmem, sig)] = shape;
     const std::string path = cache_path();
     for (size_t k = path.find('/', 1); k != std::string::npos; k = path.find('/', k + 1))
@@ -120,16 +125,11 @@ template <typename T> __global__ void k_fill_noise(C2<T> *p, long n, double amp)
     }
 }
 
-// Time `steps` iterations of a batch of nmem fields with shape v, replayed
-// from one captured graph (after one untimed replay), starting from a
-// noise spectrum whose field has r.m.s. about 0.1.
-int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
+// Time `steps` iterations of a batch of nmem fields with the parameters q,
+// replayed from one captured graph (after one untimed replay), starting from
+// a noise spectrum whose field has r.m.s. about 0.1.
+int time_steps(const spm_params &q, int nmem, int steps, double *us)
 {
-    spm_params q = *p;
-    q.shape = v;
-    // the shape sets the passes of the four-pass step, which are the ones the
-    // other operators (LOBPCG, SPM forces) run; the two-pass step has its own
-    q.step_passes = 4;
     spm_ctx *h = nullptr;
     RCHK(spm_ctx_create(&q, nmem, 2L * steps + 2, &h));
     Ctx &c = h->c;
@@ -177,6 +177,26 @@ int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
     spm_ctx_free(h);
     return rc;
 }
+
+// the shape sets the passes of the four-pass step, which are the ones the
+// other operators (LOBPCG, SPM forces) run; the two-pass step has its own
+int time_shape(const spm_params *p, int nmem, int steps, int v, double *us)
+{
+    spm_params q = *p;
+    q.shape = v;
+    q.step_passes = 4;
+    q.step_variant = 0;
+    return time_steps(q, nmem, steps, us);
+}
+
+int time_step_variant(const spm_params *p, int nmem, int steps, int v, double *us)
+{
+    spm_params q = *p;
+    q.shape = p->shape < 0 ? 0 : p->shape; // the two-pass step does not depend on it
+    q.step_passes = 2;
+    q.step_variant = SPM_STEP_EXPLICIT | v;
+    return time_steps(q, nmem, steps, us);
+}
 } // namespace
 
 int spm_resolve_shape(const spm_params *p, int nmem, int *shape)
@@ -192,6 +212,21 @@ int spm_resolve_shape(const spm_params *p, int nmem, int *shape)
         }
     }
     return spm_tune(p, nmem, 20, shape, nullptr);
+}
+
+int spm_resolve_step(const spm_params *p, int nmem, int *variant)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        load_cache();
+        auto it = g_table.find(key(step_arch(), p->N, p->precision, nmem, step_sig()));
+        if (it != g_table.end() && it->second >= 0 && it->second < SPM_STEP_VARIANTS &&
+            (it->second & SPM_STEP_U_MASK) != SPM_STEP_U_MASK) {
+            *variant = it->second;
+            return SPM_OK;
+        }
+    }
+    return spm_tune_step(p, nmem, 20, variant, nullptr);
 }
 
 extern "C" {
@@ -231,7 +266,32 @@ int spm_tune(const spm_params *p, int nmem, int steps, int *best, double *us_per
     *best = bi;
     std::lock_guard<std::mutex> lk(g_mu);
     load_cache();
-    record(arch(), p->N, p->precision, nmem, bi, bt);
+    record(arch(), p->N, p->precision, nmem, bi, bt, table_sig(p->N));
+    return SPM_OK;
+}
+
+int spm_tune_step(const spm_params *p, int nmem, int steps, int *best, double *us_per_variant)
+{
+    if (!p || !best) return spm_fail(SPM_EINVAL, "spm_tune_step: null argument");
+    if (nmem < 1 || steps < 1) return spm_fail(SPM_EINVAL, "spm_tune_step: nmem >= 1, steps >= 1");
+    if (p->N != 64 && p->N != 128 && p->N != 256)
+        return spm_fail(SPM_ENOTSUP, "spm_tune_step: the two-pass step exists for N = 64, 128 and 256 only");
+    int bi = -1;
+    double bt = 0.0;
+    for (int v = 0; v < SPM_STEP_VARIANTS; ++v) {
+        double us = 0.0;
+        const int rc = time_step_variant(p, nmem, steps, v, &us);
+        if (rc) return spm_fail(rc, "spm_tune_step: variant " + std::to_string(v) + ": " + spm_last_error());
+        if (us_per_variant) us_per_variant[v] = us;
+        if (bi < 0 || us < bt) {
+            bi = v;
+            bt = us;
+        }
+    }
+    *best = bi;
+    std::lock_guard<std::mutex> lk(g_mu);
+    load_cache();
+    record(step_arch(), p->N, p->precision, nmem, bi, bt, step_sig());
     return SPM_OK;
 }
 

@@ -151,6 +151,28 @@ def test_tuner_records_and_resolves(spm, torch, tmp_path, monkeypatch):
     assert len(cache.read_text().strip().splitlines()) == 1  # no second measurement
 
 
+def test_step_tuner_records_and_resolves(spm, torch, tmp_path, monkeypatch):
+    """spm_tune_step times every two-pass step variant, records the fastest
+    under <arch>/sj, and step_variant -1 resolves from that record."""
+    cache = tmp_path / "shapes.txt"
+    monkeypatch.setenv("SPM_TUNE_CACHE", str(cache))
+    best, us = spm.tune_step(64, spm.FP32, nmem=2, steps=4)
+    assert 0 <= best < spm.STEP_VARIANTS and len(us) == spm.STEP_VARIANTS and min(us) > 0
+    assert us[best] == min(us)
+    line = cache.read_text().strip().splitlines()[-1].split()
+    assert line[0].endswith("/sj")
+    assert line[1:5] == ["64", "1", "2", str(best)]  # N, precision, members (bucketed), flags
+    with spm.Batch(64, spm.FP32, capacity=2, hist_cap=4, step_variant=-1) as b:
+        assert b.capacity == 2
+    assert len(cache.read_text().strip().splitlines()) == 1  # no second measurement
+
+
+def test_step_variant_rejected(spm, torch):
+    for bad in (3, 8 | 6, 8 | 7, 16, -2):
+        with pytest.raises(spm.SpmError):
+            spm.Grid(64, spm.FP64, step_variant=bad)
+
+
 @pytest.fixture(scope="module")
 def sweep():
     import importlib

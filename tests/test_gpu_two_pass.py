@@ -47,6 +47,59 @@ def test_two_pass_gradflow_matches_oracle(spm, po, seeds, torch, N):
     assert np.abs(out - uR.real).max() <= 1e-10
 
 
+def _variants(spm):
+    # every flag set spm_tune_step chooses from
+    return [spm.STEP_EXPLICIT | f for f in range(spm.STEP_VARIANTS)]
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_step_variants_match_oracle(spm, po, seeds, torch, N):
+    """Every two-pass step variant (state preloaded or gathered, cache hints by
+    size, off or on), FP64: 40 iterations through graph replays, per-step E,
+    ||grad F||, residual and the final field against the oracle (get_min_u_F,
+    LPsrc.cpp:40-117)."""
+    steps = 40
+    phi = seeds.icosahedral_nucleus(N, seed=0)
+    o = po.Oracle(N)
+    uR = phi.astype(np.complex128)
+    uF = o.my_fft(uR)
+    it, E, res, gn = o.get_min_u_F(uR, uF, steps, 1e-16, gnorm=True)
+    for v in _variants(spm):
+        print(f"[case] step variant {v} N={N} vs oracle", flush=True)
+        with spm.Grid(N, 0, step_variant=v, **TWO) as g:
+            pt = torch.as_tensor(phi).cuda()
+            itg, hist = g.gradflow(pt, steps, 1e-16, 0.0, check_every=32)
+            out = pt.cpu().numpy()
+        assert itg == it == steps, v
+        assert np.all(np.abs(hist[:, 0] - E) <= 1e-12 * np.abs(E)), v
+        assert np.all(np.abs(hist[:, 1] - gn) <= 1e-9 * gn + 1e-14), v
+        assert np.all(np.abs(hist[:, 2] - res) <= 1e-8 * res + 1e-15), v
+        assert np.abs(out - uR.real).max() <= 1e-10, v
+
+
+@pytest.mark.parametrize("N", NS)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_step_variants_compute_the_same(spm, seeds, torch, N, prec):
+    """The variants differ only in when the state is loaded and in cache hints:
+    the history and the field equal the built-in variant's bit for bit."""
+    steps = 24
+    dt = torch.float64 if prec == 0 else torch.float32
+    phi = _dev_seed(seeds, torch, N, "bcc", 5, dt)
+    ref = None
+    for v in [0] + _variants(spm):
+        print(f"[case] step variant {v} N={N} prec={prec}", flush=True)
+        with spm.Grid(N, prec, step_variant=v, **TWO) as g:
+            t = phi.clone()
+            it, hist = g.gradflow(t, steps, -1.0, 0.0, check_every=16)
+            got = (it, hist.copy(), t.cpu().numpy())
+        if ref is None:
+            ref = got
+            continue
+        assert got[0] == ref[0] == steps, v
+        assert np.array_equal(got[1], ref[1]), v
+        assert np.array_equal(got[2], ref[2]), v
+
+
 @pytest.mark.parametrize("N", NS)
 @pytest.mark.parametrize("prec", [0, 1])
 def test_two_pass_equals_four_pass(spm, seeds, torch, N, prec):
