@@ -277,8 +277,9 @@ int spm_batch_release(spm_batch *b, int slot);
 /* Shapes available for N: count, and (if tpp != NULL) 3 ints per shape. */
 int spm_shapes(int N, int *count, int *tpp, int max_shapes);
 /* Time `steps` gradient-flow iterations of a batch of nmem members for every
- * shape of p->N in p->precision; *best <- the fastest; us_per_step (host,
- * may be NULL, one per shape) <- the times.  The result is recorded in the
+ * shape of p->N in p->precision (two sweeps over the shapes, the faster time
+ * of each kept); *best <- the fastest; us_per_step (host, may be NULL, one per
+ * shape) <- the times.  The result is recorded in the
  * tuning cache (the file named by SPM_TUNE_CACHE, default
  * $HOME/.cache/spm-iqc/shapes.txt), which spm_create and spm_batch_create
  * read for shape -1. */

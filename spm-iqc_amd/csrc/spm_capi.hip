@@ -237,10 +237,17 @@ int spm_check_params(const spm_params *p, const char *who)
     return SPM_OK;
 }
 
-// The built-in two-pass step variant: FP32 gathers U where it is used (its
-// pass 2 then runs at 88 registers; 11 us of 124 at 256^3 on MI355X), FP64
-// preloads it (gathering costs FP64 a quarter of its step at 256^3).
-static int builtin_step_variant(int, int prec) { return prec == SPM_FP32 ? SPM_STEP_GATHER_U : 0; }
+// The built-in two-pass step variant, from spm_tune_step on MI355X: U gathered
+// where it is used (FP32 pass 2 then runs at 88 registers: 113 us per step at
+// 256^3 against 123 preloaded), except FP64 at 256^3, whose pass 2 holds 2
+// waves per SIMD either way and preloads (194 us against 215 gathered); one
+// FP64 256^3 field keeps U cached (W + U = 270 MB, just over the last-level
+// cache: 194 us against 199 with the hints).  Batches follow the size rule.
+static int builtin_step_variant(int N, int prec, int nmem)
+{
+    if (prec == SPM_FP32 || N != 256) return SPM_STEP_GATHER_U;
+    return nmem == 1 ? SPM_STEP_U_CACHED : 0;
+}
 
 static int ensure_hist(spm_ctx *h, long n)
 {
@@ -313,7 +320,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     c.shape = shape;
     c.gf2 = p->step_passes == 2 || (p->step_passes == 0 && (c.N == 64 || c.N == 128 || c.N == 256));
     if (p->step_variant == 0 || !c.gf2) {
-        c.sjv = builtin_step_variant(c.N, c.prec);
+        c.sjv = builtin_step_variant(c.N, c.prec, nmem);
     } else if (p->step_variant < 0) { // the tuned variant for this size, precision and batch
         const int rc = spm_resolve_step(p, nmem, &c.sjv);
         if (rc) {

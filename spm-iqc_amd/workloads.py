@@ -5,11 +5,14 @@ command per kernel family, printing its own timing.
     python spm-iqc_amd/workloads.py lobpcg   [--precision ...] [--iters 5]
     python spm-iqc_amd/workloads.py spm      [--precision ...] [--iters 20]
     python spm-iqc_amd/workloads.py hisd     [--precision ...] [--iters 20]
+    python spm-iqc_amd/workloads.py variants [--precision ...] [--steps 50]
 
 gradflow: get_min_u_F iterations on the icosahedral nucleus seed (LPsrc.cpp:40-117).
 lobpcg:   hiosd_lobpcg_inip iterations, block 8, on the same field (LOBPCG.cpp:207-487).
 spm:      Spring_Pairing iterations from phi -/+ 0.01 v (SPM.cpp:170-263).
 hisd:     hiosd_lobpcg_sieh steps, k = 1, from phi + 0.01 v (HISD.cpp:13-30, 86-485).
+variants: the two-pass step's variants timed by spm_tune_step (graph-replayed steps of a
+          noise spectrum), the fastest recorded in the tuning cache.
 """
 from __future__ import annotations
 
@@ -110,9 +113,20 @@ def hisd(N, prec, iters, shape, passes=0):
     print("hisd N %d prec %d: %d steps, %.2f ms/step, res %.3e" % (N, prec, it, dt / max(it, 1) * 1e3, res))
 
 
+def variants(N, prec, steps):
+    best, us = spm.tune_step(N, prec, nmem=1, steps=steps)
+    names = []
+    for f in range(spm.STEP_VARIANTS):
+        u = {0: "U hint by size", spm.STEP_U_CACHED: "U cached", spm.STEP_U_STREAM: "U streamed"}[f & 6]
+        names.append(("U gathered" if f & spm.STEP_GATHER_U else "U preloaded") + ", " + u)
+    for f in range(spm.STEP_VARIANTS):
+        print("variants N %d prec %d flags %d (%s): %.1f us/step%s" % (N, prec, f, names[f], us[f],
+                                                                     "  <- fastest" if f == best else ""))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("workload", choices=("gradflow", "lobpcg", "spm", "hisd"))
+    ap.add_argument("workload", choices=("gradflow", "lobpcg", "spm", "hisd", "variants"))
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--precision", choices=("fp64", "fp32", "both"), default="both")
     ap.add_argument("--steps", type=int, default=50)
@@ -129,6 +143,8 @@ def main(argv=None):
             gradflow(a.N, p, a.steps, want[p], a.step_passes)
         elif a.workload == "lobpcg":
             lobpcg(a.N, p, a.iters or 5, want[p], a.step_passes)
+        elif a.workload == "variants":
+            variants(a.N, p, a.steps)
         elif a.workload == "spm":
             saddle(a.N, p, a.iters or 20, want[p], a.step_passes)
         else:
