@@ -317,6 +317,74 @@ __global__ void __launch_bounds__(256) k_gram_xy(FieldPtrs X, int mx, FieldPtrs 
     }
 }
 
+// k_gram_xy for X and Y of at most XYS fields each (the sweeps of HiSD with one
+// or two directions), where an MFMA tile would keep only a few of its rows
+// loading: a streaming VALU pass, four points per thread, every field read
+// once.  Partials in k_gram_xy<T, 1, 1>'s layout (entry a * 16 + b of the
+// 16 x 16 tile: rows X and the ones row mx, columns [X, Y]), so the same
+// reduction finishes it.
+constexpr int XYS = 3;
+template <typename T>
+__global__ void __launch_bounds__(256) k_gram_xy_small(FieldPtrs X, int mx, FieldPtrs Y, int my, long R,
+                                                      double *__restrict__ part)
+{
+    constexpr int NE = XYS * (2 * XYS) + 2 * XYS; // X^T [X Y] and 1^T [X Y]
+    __shared__ double sh[4][NE];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) acc[e] = 0.0;
+    auto load4 = [&](const void *f, long r, double (&v)[4]) {
+        if constexpr (sizeof(T) == 8) {
+            const double2 u = *((const double2 *)f + r / 2), w = *((const double2 *)f + r / 2 + 1);
+            v[0] = u.x; v[1] = u.y; v[2] = w.x; v[3] = w.y;
+        } else {
+            const float4 u = *((const float4 *)f + r / 4);
+            v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+        }
+    };
+    const long stride = (long)gridDim.x * blockDim.x * 4;
+#pragma unroll 1
+    for (long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4; r < R; r += stride) {
+        double c[2 * XYS][4]; // columns [X, Y]
+#pragma unroll
+        for (int b = 0; b < 2 * XYS; ++b) {
+            const bool isx = b < XYS;
+            const int bi = isx ? b : b - XYS;
+            if (bi < (isx ? mx : my)) load4(isx ? X.p[bi] : Y.p[bi], r, c[b]);
+            else c[b][0] = c[b][1] = c[b][2] = c[b][3] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < XYS; ++a)
+#pragma unroll
+            for (int b = 0; b < 2 * XYS; ++b)
+                acc[a * 2 * XYS + b] += ((c[a][0] * c[b][0] + c[a][1] * c[b][1]) + c[a][2] * c[b][2]) + c[a][3] * c[b][3];
+#pragma unroll
+        for (int b = 0; b < 2 * XYS; ++b) acc[XYS * 2 * XYS + b] += ((c[b][0] + c[b][1]) + c[b][2]) + c[b][3];
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        double v = acc[e];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) sh[wid][e] = v;
+    }
+    __syncthreads();
+    // the 16 x 16 tile of this block: row a < mx (X_a) or a = mx (ones), column b < mx (X_b) or mx + b (Y_b)
+    double *pb = part + (long)blockIdx.x * 256;
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        const int a = t / 16, b = t % 16;
+        double v = 0.0;
+        const int ra = a < mx ? a : (a == mx ? XYS : -1);
+        const int cb = b < mx ? b : (b - mx < my ? XYS + (b - mx) : -1);
+        if (ra >= 0 && cb >= 0) {
+            const int e = ra * 2 * XYS + cb;
+            v = ((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e];
+        }
+        pb[t] = v;
+    }
+}
+
 constexpr int COMBINE_NO = 8; // outputs per k_combine launch
 
 // Operands of one k_combine launch, passed by value (read with uniform scalar
@@ -524,12 +592,14 @@ int block_gram_xy(spm_ctx *h, const void *const *X, int mx, const void *const *Y
     for (int a = 0; a < mx; ++a) xp.p[a] = X[a];
     for (int b = 0; b < my; ++b) yp.p[b] = Y[b];
     const int ta = (mx + 1 + 15) / 16, tb = std::max(ta, (mx + my + 15) / 16);
-    const int nblk = (int)std::min<long>(GRAM_BLOCKS, std::max<long>(1, c.R / 64));
+    const bool small = mx <= XYS && my <= XYS;
+    const int nblk = (int)std::min<long>(small ? GRAM_BLOCKS_SMALL : GRAM_BLOCKS, std::max<long>(1, c.R / 64));
     double *part = gram_part(h);
 #define XY_LAUNCH(T, A_, B_)                                                                                   \
     hipLaunchKernelGGL((k_gram_xy<T, A_, B_>), dim3(nblk), dim3(256), 0, c.stream, xp, mx, yp, my, 1, c.R, part)
 #define XY_BY_TILES(T)                                                                                         \
-    if (ta == 1 && tb == 1) XY_LAUNCH(T, 1, 1);                                                                \
+    if (small) hipLaunchKernelGGL(k_gram_xy_small<T>, dim3(nblk), dim3(256), 0, c.stream, xp, mx, yp, my, c.R, part); \
+    else if (ta == 1 && tb == 1) XY_LAUNCH(T, 1, 1);                                                           \
     else if (ta == 1 && tb == 2) XY_LAUNCH(T, 1, 2);                                                           \
     else if (ta == 1) XY_LAUNCH(T, 1, 3);                                                                      \
     else if (tb == 2) XY_LAUNCH(T, 2, 2);                                                                      \
