@@ -103,22 +103,35 @@ def test_saddle_drivers_match_oracle(driver):
     through the compat layer, in the {110} cell at 16^3: u_R_local is the index-1
     saddle Spring_Pairing reaches, option1 its lowest 4 modes.  The oracle runs
     hiosd_lobpcg_sieh(_V0) with the driver's perturbation, directions, null space and
-    parameters from the same inputs.  Both must end at the same energy, and at the
-    same field to the driver's own tolerance; the drivers that read seed files
+    parameters from the same inputs.  Both must end at the same energy, the oracle
+    must certify the GPU's end point as critical to the driver's tolerance, and the
+    two end points may differ only as far as their stopping tests allow; the drivers
+    that read seed files
     (FCC, HPC) find them in tests/golden/initData.
 
     The HiSD drivers climb back to the saddle along its one unstable mode, which V
     must match.  The GOSD drivers' null space V0 is option1's first three modes,
     the saddle's unstable mode among them, so they relax to the ordered minimum
     below it, where V ends in the (degenerate) translation modes: only the field
-    and the energy are unique there."""
+    (up to a translation) and the energy are unique there."""
     exe = os.path.join(ROOT, "tests", "compat", "compat_parity")
     out = _run([exe, "drivers", driver, "16"], cwd=os.path.join(ROOT, "tests", "golden"))
     res = json.loads([l for l in out.splitlines() if l.startswith("{")][-1])
     print(res, flush=True)
     assert res["res_oracle"] < res["epsf"] and res["iters_oracle"] < res["maxiter"]
     assert res["res"] < res["epsf"] * 1.01
-    assert res["x"] < max(1e-8, res["epsf"])
+    # the oracle certifies the GPU's end point as a critical point to the driver's tolerance
+    assert res["res_oracle_at_gpu"] < res["epsf"] * 1.01
+    # Both runs stop anywhere inside the ball ||ngrad|| < epsf, so their end points
+    # may differ by dx along directions the Hessian barely resists (the minimum's
+    # translations and soft modes): to first order H dx is the difference of the two
+    # end gradients, so ||H dx|| <= 2 epsf, and dx itself must stay small (one
+    # critical point, not two).
+    assert res["hdx"] <= 2.02 * res["epsf"]
+    assert res["x"] < 1e-4
+    if driver.startswith("critical_point_HISD"):
+        # climbing back to the saddle the two runs also agree pointwise to epsf
+        assert res["x"] < max(1e-8, res["epsf"])
     # at a point where ||grad F|| < epsf, E moves by at most about epsf |dx| with the field
     assert abs(res["E"] - res["E_oracle"]) <= 1e-10 * abs(res["E_oracle"]) + res["epsf"] * res["x"]
     if driver.startswith("critical_point_HISD"):
