@@ -128,6 +128,30 @@ def time_gpu(spm, torch, N, prec, steps, warmup, phi0, shape=0, passes=0, warm_s
                 warm_steps=warm_steps)
 
 
+# One cell of the {110} shell in the reference's metric (k = n / L with L = sqrt 2):
+# its 12 wave vectors sit at |k| = 1, the minimum of G, and a BCC-seeded
+# relaxation reaches the ordered minimum by its tolerance, not by a step cap.
+CELL = dict(L=2 ** 0.5, kscale=1.0, Tau=-0.0061, Gamma=0.30, dt=0.3)
+
+
+def converged_relax(N, gn_tol=1e-8, switch_tol=1e-6, max_iter=100000):
+    """The mixed pipeline's accuracy at a converged state: relaxation of the BCC
+    {110} seed in the cell to ||grad F|| <= gn_tol, in FP64 and mixed (FP32 steps
+    to switch_tol, then FP64 steps to gn_tol).  Returns both fields (host) and
+    the runs' figures."""
+    pipeline = importlib.import_module("spm-iqc_amd.pipeline")
+    seeds = importlib.import_module("spm-iqc_amd.seeds")
+    phi0 = seeds.crystal(N, "bcc", L=CELL["L"], kscale=CELL["kscale"], amp=0.1, seed=0)
+    runs, fields = {}, {}
+    for mode in ("fp64", "mixed"):
+        phi, info = pipeline.relax(N, phi0, mode, gn_tol=gn_tol, max_iter=max_iter, polish=0, params=CELL,
+                                   switch_tol=switch_tol if mode == "mixed" else None)
+        fields[mode] = phi.cpu().numpy()
+        runs[mode] = {"E": info["E"], "grad_norm": info["gn"], "steps_fp32": info["iters_fp32"],
+                      "steps_fp64": info["iters_fp64"], "seconds": round(info["t"], 4)}
+    return fields, runs
+
+
 def cpu_baseline(po, N, phi0, budget_s=20.0, check_fields=None):
     """The oracle (the reference's FP64 algorithm, OpenMP threads) on a bounded
     sample of the same workload: gradient-flow steps at the same N.  As the
@@ -193,6 +217,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests of the serving stream (p50/p99 latency); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=12.0)
+    ap.add_argument("--converged-N", type=int, default=256,
+                    help="grid of the mixed-vs-FP64 relaxation to a converged state (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the workload is one 256^3 field); "
@@ -235,6 +261,10 @@ def main():
     ach64 = B64 * sps64
     ach32 = B32 * sps32
 
+    conv_fields, conv = None, None
+    if args.converged_N > 0:
+        conv_fields, conv = converged_relax(args.converged_N)
+
     serving = None
     if args.serve_requests > 0:
         serving = serve_stream(args.serve_requests, args.serve_rate)
@@ -251,6 +281,21 @@ def main():
     # the mixed pipeline ends with an FP64 polish, spm-iqc_amd/pipeline.py)
     mixed_err = {"E_fp64": r64["E_after"], "E_mixed": r32["E_after"],
                  "E_rel_mixed_vs_fp64": abs(r32["E_after"] - r64["E_after"]) / abs(r64["E_after"])}
+    if conv is not None:
+        e64, emx = conv["fp64"]["E"], conv["mixed"]["E"]
+        conv.update({
+            "workload": "BCC {110} seed (amplitude 0.1, seeded phases) in one {110} cell, L = sqrt 2, k = n/L, "
+                        "%d^3, relaxed to |grad F| <= 1e-8; mixed: FP32 steps to |grad F| 1e-6, then FP64 steps"
+                        % args.converged_N,
+            "E_rel_mixed_vs_fp64": abs(emx - e64) / abs(e64),
+            "field_max_diff": float(np.abs(conv_fields["mixed"] - conv_fields["fp64"]).max()),
+            "time_ratio_fp64_vs_mixed": round(conv["fp64"]["seconds"] / conv["mixed"]["seconds"], 3)})
+        if not args.no_cpu:
+            o = po.Oracle(args.converged_N, L=CELL["L"], kscale=CELL["kscale"])
+            eo = {m: o.energy_of(conv_fields[m]) for m in ("fp64", "mixed")}
+            o.close()
+            conv.update({"E_oracle_at_fp64_field": eo["fp64"], "E_oracle_at_mixed_field": eo["mixed"],
+                         "E_rel_mixed_vs_fp64_oracle": abs(eo["mixed"] - eo["fp64"]) / abs(eo["fp64"])})
     if E_oracle:
         mixed_err.update({
             "E_oracle_at_fp64_field": E_oracle["fp64"], "E_oracle_at_mixed_field": E_oracle["fp32"],
@@ -282,7 +327,8 @@ def main():
                   "ms_per_step": round(r32["event_ms"] / args.steps, 4),
                   "speedup_vs_fp64": round(sps32 / sps64, 3),
                   "roofline_frac": round(ach32 / HBM_PEAK, 4),
-                  "energy_error": mixed_err},
+                  "energy_error": mixed_err,
+                  "converged": conv},
         "roofline": {"bound": "hbm",
                      "kernel": ("gradient-flow step (2 launches: j stage B + i-axis FFT + update + inverse + B^-1; "
                                 "j stage A^-1 + k-axis c2r + f(phi) + r2c + A)" if passes == 2 else
