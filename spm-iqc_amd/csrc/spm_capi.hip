@@ -339,6 +339,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     c.g->geometry(c.nb_rows, c.nb_j, c.nb_i, c.NHP);
     int rc = SPM_OK;
     auto bail = [&](int code) {
+        if (h->own) (void)hipStreamSynchronize(h->own); // uploads in flight read host buffers of this frame
         free_all(h);
         delete h;
         return code;
@@ -346,23 +347,26 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     if (hipError_t e = hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking); e != hipSuccess)
         return bail(spm_fail(SPM_EHIP, std::string("spm_create: hipStreamCreate: ") + hipGetErrorString(e)));
     c.stream = h->own;
+    // Every upload and fill below is ordered on the context's own stream, which
+    // is synchronised before returning: the legacy stream (hipMemcpy, hipMemset)
+    // is not ordered with the non-blocking streams the context's work runs on,
+    // so a fill queued there could land after the first kernels.  The host
+    // sources stay alive until that synchronisation.
     // twiddles exp(-2 pi i e / N)
-    {
-        std::vector<double> twd;
-        std::vector<float> twf;
-        spm_host_twiddles(c.N, twd, twf);
-        if ((rc = spm_dev_alloc(h, &c.tw, 2 * c.N * c.esz))) return bail(rc);
-        if (hipMemcpy(c.tw, c.prec == SPM_FP64 ? (void *)twd.data() : (void *)twf.data(), 2 * c.N * c.esz,
-                      hipMemcpyHostToDevice) != hipSuccess)
-            return bail(spm_fail(SPM_EHIP, "spm_create: twiddle upload failed"));
-    }
+    std::vector<double> twd;
+    std::vector<float> twf;
+    spm_host_twiddles(c.N, twd, twf);
+    if ((rc = spm_dev_alloc(h, &c.tw, 2 * c.N * c.esz))) return bail(rc);
+    if (hipMemcpyAsync(c.tw, c.prec == SPM_FP64 ? (void *)twd.data() : (void *)twf.data(), 2 * c.N * c.esz,
+                       hipMemcpyHostToDevice, c.stream) != hipSuccess)
+        return bail(spm_fail(SPM_EHIP, "spm_create: twiddle upload failed"));
     // G and preconditioner tables over |n|^2 (get_ksquare_local, get_epmckt2_local,
     // get_ikt2_local; Initialization.cpp:123-174)
+    const int h2 = c.N / 2;
+    c.ntab = 3 * h2 * h2 + 1;
+    std::vector<double> Gt(c.ntab), It(c.ntab), Bt(c.ntab), GDd(2 * c.ntab);
+    std::vector<float> GDf(2 * c.ntab);
     {
-        const int h2 = c.N / 2;
-        c.ntab = 3 * h2 * h2 + 1;
-        std::vector<double> Gt(c.ntab), It(c.ntab), Bt(c.ntab), GDd(2 * c.ntab);
-        std::vector<float> GDf(2 * c.ntab);
         for (int n2 = 0; n2 < c.ntab; ++n2) {
             Gt[n2] = spm_get_GK(spm_host_k2(c.kscale, c.L, n2));
             It[n2] = 1.0 / (Gt[n2] + std::fabs(c.Tau) + 0.02);
@@ -376,11 +380,14 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
         if ((rc = spm_dev_alloc(h, (void **)&c.Itab, sizeof(double) * c.ntab))) return bail(rc);
         if ((rc = spm_dev_alloc(h, &c.GDtab, 2 * c.esz * c.ntab))) return bail(rc);
         if ((rc = spm_dev_alloc(h, (void **)&c.Btab, sizeof(double) * c.ntab))) return bail(rc);
-        if (hipMemcpy(c.Gtab, Gt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c.Itab, It.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c.Btab, Bt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c.GDtab, c.prec == SPM_FP64 ? (const void *)GDd.data() : (const void *)GDf.data(),
-                      2 * c.esz * c.ntab, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpyAsync(c.Gtab, Gt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice, c.stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(c.Itab, It.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice, c.stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(c.Btab, Bt.data(), sizeof(double) * c.ntab, hipMemcpyHostToDevice, c.stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(c.GDtab, c.prec == SPM_FP64 ? (const void *)GDd.data() : (const void *)GDf.data(),
+                           2 * c.esz * c.ntab, hipMemcpyHostToDevice, c.stream) != hipSuccess)
             return bail(spm_fail(SPM_EHIP, "spm_create: table upload failed"));
     }
     const size_t spec = (size_t)c.C * 2 * c.esz;
@@ -392,18 +399,19 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
     if ((rc = spm_dev_alloc(h, (void **)&c.part, sizeof(double) * c.part_slot * c.nslots))) return bail(rc);
     if ((rc = spm_dev_alloc(h, (void **)&c.scal, sizeof(double) * SC_COUNT))) return bail(rc);
     if ((rc = spm_dev_alloc(h, (void **)&c.ctl, sizeof(Ctl) * nmem))) return bail(rc);
+    std::vector<double> prm(2 * nmem);
     if (nmem > 1) { // every member starts with the context's (Tau, Gamma)
         if ((rc = spm_dev_alloc(h, (void **)&c.prm, sizeof(double) * 2 * nmem))) return bail(rc);
-        std::vector<double> prm(2 * nmem);
         for (int m = 0; m < nmem; ++m) {
             prm[2 * m] = c.Tau;
             prm[2 * m + 1] = c.Gamma;
         }
-        if (hipMemcpy(c.prm, prm.data(), sizeof(double) * 2 * nmem, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpyAsync(c.prm, prm.data(), sizeof(double) * 2 * nmem, hipMemcpyHostToDevice, c.stream) !=
+            hipSuccess)
             return bail(spm_fail(SPM_EHIP, "spm_create: parameter upload failed"));
     }
     if ((rc = spm_dev_alloc(h, (void **)&c.counter, 64 * sizeof(unsigned)))) return bail(rc);
-    if (hipMemset(c.counter, 0, 64 * sizeof(unsigned)) != hipSuccess)
+    if (hipMemsetAsync(c.counter, 0, 64 * sizeof(unsigned), c.stream) != hipSuccess)
         return bail(spm_fail(SPM_EHIP, "spm_create: memset"));
     if (hipHostMalloc((void **)&h->h_pin, 4096, hipHostMallocDefault) != hipSuccess)
         return bail(spm_fail(SPM_ENOMEM, "spm_create: pinned scratch"));
@@ -413,7 +421,7 @@ int spm_ctx_create(const spm_params *p, int nmem, long hist_cap, spm_ctx **out)
         if ((rc = spm_dev_alloc(h, (void **)&c.hist, sizeof(double) * 3 * hist_cap * nmem))) return bail(rc);
         c.hist_cap = hist_cap;
     }
-    if (hipMemset(c.scal, 0, sizeof(double) * SC_COUNT) != hipSuccess)
+    if (hipMemsetAsync(c.scal, 0, sizeof(double) * SC_COUNT, c.stream) != hipSuccess)
         return bail(spm_fail(SPM_EHIP, "spm_create: memset"));
     for (int m = 0; m < nmem; ++m) // every member starts stopped
         hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, c.stream, c.ctl + m, 0, 0.0, 0.0);

@@ -16,7 +16,18 @@ Latency of a request = completion time - arrival time.  The reference
 point is the same stream served one request at a time in FP64, in arrival
 order.
 
-    python spm-iqc_amd/serve.py [--requests 500] [--rate 8] [--baseline]
+    python spm-iqc_amd/serve.py [--requests 500] [--rate 8] [--baseline] [--as-written]
+    python spm-iqc_amd/serve.py --as-written --baseline-only --out fp64.json
+    python spm-iqc_amd/serve.py --compare mixed.json fp64.json
+
+The default stream is reduced (relaxations to ||grad F|| 1e-6 or 2000 steps, 2
+LOBPCG calls, 100 Spring_Pairing iterations).  --as-written serves the
+configuration as stated: relaxations to ||grad F|| <= 1e-8 within the
+reference's 100000 steps (LPsrc.cpp:52; mixed: FP32 to 1e-6, then FP64), the
+4 lowest modes from 10 LOBPCG calls of 21 iterations (main_SPM.cpp:88-91),
+and Spring_Pairing(30000, 1e-9) (main_SPM.cpp:122).  Its one-at-a-time FP64
+reference can run as a separate process (--baseline-only) and be compared
+afterwards (--compare), without a GPU.
 """
 from __future__ import annotations
 
@@ -46,12 +57,19 @@ PARAMS = dict(L=L, kscale=KSCALE, Tau=-0.0061, Gamma=0.30, dt=0.3)
 # ---------------------------------------------------------------------------
 # workload
 # ---------------------------------------------------------------------------
+# the stream as the configuration states it (see the module docstring)
+AS_WRITTEN = dict(relax_tol=1e-8, relax_iters=100000, switch_tol=1e-6, modes_calls=10, spm_iters=30000,
+                  spm_tol=1e-9)
+
+
 def make_requests(n=500, rate=8.0, seed=0, relax_sizes=(64, 128, 256), modes_sizes=(64, 128), spm_size=128,
-                  relax_tol=1e-6, relax_iters=2000, modes_calls=2, spm_iters=100):
+                  relax_tol=1e-6, relax_iters=2000, modes_calls=2, spm_iters=100, switch_tol=None, spm_tol=1e-9):
     """Poisson arrivals (rate per second) with a seeded mix: 60% relax (seeds
     round-robin over the four families, (Tau, Gamma) from the sweep grid), 25%
     lowest-4-mode LOBPCG on a relaxed state, 15% SPM from a relaxed
-    icosahedral nucleus offset along its first mode."""
+    icosahedral nucleus offset along its first mode.  switch_tol: the
+    ||grad F|| at which a mixed relaxation hands over from FP32 to FP64 (None:
+    at its own tolerance)."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n))
     u = rng.uniform(size=n)
@@ -62,12 +80,12 @@ def make_requests(n=500, rate=8.0, seed=0, relax_sizes=(64, 128, 256), modes_siz
             Tau, Gamma = grid[int(rng.integers(len(grid)))]
             reqs.append(dict(id=i, t=float(t[i]), kind="relax", N=int(rng.choice(relax_sizes)),
                              Tau=float(Tau), Gamma=float(Gamma), seed_kind=sweep.KINDS[i % 4], seed=i,
-                             tol=relax_tol, max_iter=relax_iters))
+                             tol=relax_tol, max_iter=relax_iters, switch_tol=switch_tol))
         elif u[i] < 0.85:
             reqs.append(dict(id=i, t=float(t[i]), kind="modes", N=int(rng.choice(modes_sizes)), seed=i,
                              calls=modes_calls))
         else:
-            reqs.append(dict(id=i, t=float(t[i]), kind="saddle", N=spm_size, seed=i, iters=spm_iters))
+            reqs.append(dict(id=i, t=float(t[i]), kind="saddle", N=spm_size, seed=i, iters=spm_iters, tol=spm_tol))
     return reqs
 
 
@@ -143,7 +161,7 @@ def run_saddle(r, states, mixed):
     st = states[r["N"]]
     n = r["iters"]
     p0, p1, info = pipeline.saddle(r["N"], st["phi"], st["v1"], "mixed" if mixed else "fp64", offset=0.01,
-                                   maxiter=n, tol=1e-9, spm_polish=max(n // 5, 1), params=PARAMS)
+                                   maxiter=n, tol=r.get("tol", 1e-9), spm_polish=max(n // 5, 1), params=PARAMS)
     return dict(E=info["E"], res=info["res"], iters=info["iters"])
 
 
@@ -155,9 +173,15 @@ def run_relax_single(r):
     return dict(iters=it, E=float(hist[-1, 0]), gn=float(hist[-1, 1]))
 
 
+# Worker requests start shortest expected service first: an eigenmode solve
+# (a few hundred LOBPCG iterations) before a saddle search (thousands of
+# Spring_Pairing iterations), each kind in arrival order.
+PRIORITY = {"modes": 0, "saddle": 1}
+
+
 class _Workers:
     def __init__(self, n, states, mixed):
-        self.q = queue.Queue()
+        self.q = queue.PriorityQueue()  # (priority, arrival, id, request); None-request entries stop a thread
         self.done = queue.Queue()
         self.states, self.mixed = states, mixed
         self.threads = [threading.Thread(target=self._loop, daemon=True) for _ in range(n)]
@@ -168,7 +192,7 @@ class _Workers:
         torch = spm._torch()
         s = torch.cuda.Stream()
         while True:
-            r = self.q.get()
+            r = self.q.get()[3]
             if r is None:
                 return
             try:
@@ -180,9 +204,12 @@ class _Workers:
             except Exception as e:  # reported with the request
                 self.done.put((r, None, repr(e)))
 
+    def put(self, r):
+        self.q.put((PRIORITY[r["kind"]], r["t"], r["id"], r))
+
     def close(self):
-        for _ in self.threads:
-            self.q.put(None)
+        for i, _ in enumerate(self.threads):
+            self.q.put((len(PRIORITY), float("inf"), i, None))
         for t in self.threads:
             t.join()
 
@@ -223,7 +250,8 @@ def serve(reqs, states, mixed=True, capacity=None, chunk=32, workers=4, budget_b
             nxt += 1
             if r["kind"] == "relax":
                 phi = relax_seed(r, torch.float32 if mixed else torch.float64)
-                stages[(r["N"], first)].queue.append((r, phi, r["max_iter"], r["tol"]))
+                tol = max(r["tol"], r.get("switch_tol") or 0.0) if mixed else r["tol"]
+                stages[(r["N"], first)].queue.append((r, phi, r["max_iter"], tol))
             else:
                 waiting.append(r)
         # admission of worker requests by device-memory footprint
@@ -234,7 +262,7 @@ def serve(reqs, states, mixed=True, capacity=None, chunk=32, workers=4, budget_b
             r = waiting.pop(0)
             in_flight[r["id"]] = fp
             r["t_admit"] = time.perf_counter() - t0
-            pool.q.put(r)
+            pool.put(r)
         # batches: admit, advance, retire
         for s in stages.values():
             s.admit(lambda req: req.__setitem__("t_admit", time.perf_counter() - t0))
@@ -299,6 +327,7 @@ def serve_one_at_a_time(reqs, states, verbose=False):
     recs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    shown = 0.0
     for r in sorted(reqs, key=lambda r: r["t"]):
         now = time.perf_counter() - t0
         if r["t"] > now:
@@ -314,7 +343,8 @@ def serve_one_at_a_time(reqs, states, verbose=False):
         rec = dict(id=r["id"], kind=r["kind"], N=r["N"], t=r["t"], t_done=t_done, latency=t_done - r["t"])
         rec.update(out)
         recs.append(rec)
-        if verbose and len(recs) % 50 == 0:
+        if verbose and (len(recs) % 50 == 0 or t_done - shown >= 10.0):
+            shown = t_done
             print("  one at a time: %d / %d at t = %.1f s" % (len(recs), len(reqs), t_done), flush=True)
     wall = time.perf_counter() - t0
     recs.sort(key=lambda x: x["id"])
@@ -379,23 +409,45 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--baseline", action="store_true", help="also serve one request at a time in FP64")
+    ap.add_argument("--baseline-only", action="store_true", help="serve one request at a time in FP64 only")
+    ap.add_argument("--as-written", action="store_true",
+                    help="relaxations to 1e-8 within 100000 steps, 10 LOBPCG calls, Spring_Pairing(30000, 1e-9)")
+    ap.add_argument("--compare", nargs=2, metavar=("MIXED_JSON", "FP64_JSON"),
+                    help="compare the records of two earlier runs (no GPU) and print the differences")
     ap.add_argument("--out", default=None)
     ap.add_argument("--quiet", action="store_true")
     a = ap.parse_args(argv)
-    reqs = make_requests(a.requests, a.rate, a.seed)
+    if a.compare:
+        runs = []
+        for fn in a.compare:
+            with open(fn) as f:
+                runs.append(json.load(f))
+        recs = runs[0]["records"]
+        base = runs[1]["records_fp64"] or runs[1]["records"]
+        tol = runs[0].get("relax_tol", 1e-6)
+        out = {"mixed_vs_fp64": compare(recs, base, tol)}
+        print(json.dumps(out), flush=True)
+        return out
+    kw = dict(AS_WRITTEN) if a.as_written else {}
+    reqs = make_requests(a.requests, a.rate, a.seed, **kw)
     states = prepare_states(reqs, verbose=not a.quiet)
-    recs, st = serve(reqs, states, mixed=True, chunk=a.chunk, workers=a.workers, verbose=not a.quiet)
-    out = {"rate_rps": a.rate, "requests": a.requests, "seed": a.seed, "mixed_batched": st}
-    if not a.quiet:
-        print(json.dumps({"mixed_batched": st}), flush=True)
-    if a.baseline:
+    out = {"rate_rps": a.rate, "requests": a.requests, "seed": a.seed, "as_written": a.as_written,
+           "relax_tol": kw.get("relax_tol", 1e-6)}
+    recs, base = None, None
+    if not a.baseline_only:
+        recs, st = serve(reqs, states, mixed=True, chunk=a.chunk, workers=a.workers, verbose=not a.quiet)
+        out["mixed_batched"] = st
+        if not a.quiet:
+            print(json.dumps({"mixed_batched": st}), flush=True)
+    if a.baseline or a.baseline_only:
         base, sb = serve_one_at_a_time(reqs, states, verbose=not a.quiet)
         out["one_at_a_time_fp64"] = sb
-        out["mixed_vs_fp64"] = compare(recs, base)
+        if recs is not None:
+            out["mixed_vs_fp64"] = compare(recs, base, out["relax_tol"])
     print(json.dumps({k: v for k, v in out.items()}), flush=True)
     if a.out:
         with open(a.out, "w") as f:
-            json.dump(dict(out, records=recs, records_fp64=base if a.baseline else None), f, indent=1)
+            json.dump(dict(out, records=recs, records_fp64=base), f, indent=1)
     return out
 
 

@@ -62,3 +62,54 @@ def test_mixed_relax_against_one_at_a_time_fp64(serve, stream):
             both += 1
             assert abs(m["E"] - f["E"]) <= 1e-9, (r, m, f)
     assert both >= 2
+
+
+def test_worker_requests_identical_under_concurrent_load(serve, stream, spm):
+    """Eigenmode and saddle requests create their contexts while other threads
+    run theirs and the default stream is busy: every result must equal the one
+    the same request gives alone, bit for bit.  (Context set-up is ordered on
+    the context's own stream; a fill left on the legacy stream could land after
+    the first kernels of a non-blocking stream and change or poison a result.)"""
+    import threading
+
+    torch = spm._torch()
+    reqs, states = stream
+    work = [next(r for r in reqs if r["kind"] == k) for k in ("modes", "saddle")]
+    work += [dict(r, id=1000 + r["id"]) for r in work]  # each twice, at the same time
+
+    def run(r):
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            out = serve.run_modes(r, states, True) if r["kind"] == "modes" else serve.run_saddle(r, states, True)
+            s.synchronize()
+        return out
+
+    alone = {r["id"]: run(r) for r in work}
+    got, errs = {}, []
+
+    def worker(r):
+        try:
+            got[r["id"]] = run(r)
+        except Exception as e:  # reported below
+            errs.append(repr(e))
+
+    big = torch.ones(1 << 26, device="cuda")
+    threads = [threading.Thread(target=worker, args=(r,)) for r in work]
+    for t in threads:
+        t.start()
+    n = 0
+    while any(t.is_alive() for t in threads):  # keep the default stream busy meanwhile
+        big.mul_(1.0000001)
+        n += 1
+        if n % 64 == 0:
+            torch.cuda.current_stream().synchronize()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    assert not errs, errs
+    for r in work:
+        a, m = alone[r["id"]], got[r["id"]]
+        if r["kind"] == "modes":
+            assert a["eigenvalues"] == m["eigenvalues"], (r["id"], a, m)
+        else:
+            assert np.isfinite(m["E"]) and a["E"] == m["E"] and a["res"] == m["res"], (r["id"], a, m)
