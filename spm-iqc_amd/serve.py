@@ -367,19 +367,25 @@ def compare(a, b, tol=None):
     """Differences between two served copies of the same stream (a against b).
     Relaxations: where both reached their tolerance, energies compared
     absolutely at the disordered minimum (|E| <= 1e-10) and relatively at
-    ordered ones; where a run stopped at its step budget instead, the
-    relative difference of the energy reached after the same budget."""
+    ordered ones; pairs whose energies differ by more than 1e-6 relative (1e-12
+    absolute) ended in different minima and are counted apart; where a run
+    stopped at its step budget instead, the relative difference of the energy
+    reached after the same budget."""
     conv_abs, conv_rel, capped_rel, n_conv, n_cap = [], [], [], 0, 0
-    eig, sad = [], []
+    eig, sad, other = [], [], []
     for x, y in zip(a, b):
         if x["kind"] == "relax":
             t = tol if tol is not None else 1e-6
             if x["gn"] <= t and y["gn"] <= t:
                 n_conv += 1
-                if abs(y["E"]) <= 1e-10:
-                    conv_abs.append(abs(x["E"] - y["E"]))
+                d = abs(x["E"] - y["E"])
+                if d > max(1e-12, 1e-6 * abs(y["E"])):
+                    # a different minimum: the trajectories parted near a saddle
+                    other.append(x["id"])
+                elif abs(y["E"]) <= 1e-10:
+                    conv_abs.append(d)
                 else:
-                    conv_rel.append(abs(x["E"] - y["E"]) / abs(y["E"]))
+                    conv_rel.append(d / abs(y["E"]))
             else:
                 n_cap += 1
                 capped_rel.append(abs(x["E"] - y["E"]) / max(abs(y["E"]), 1e-300))
@@ -394,7 +400,8 @@ def compare(a, b, tol=None):
     def med(v):
         return float(np.median(v)) if v else None
 
-    return {"relax_converged": n_conv, "relax_converged_E_abs_max_disordered": mx(conv_abs),
+    return {"relax_converged": n_conv, "relax_converged_other_minimum": len(other),
+            "relax_converged_other_minimum_ids": other, "relax_converged_E_abs_max_disordered": mx(conv_abs),
             "relax_converged_E_rel_max_ordered": mx(conv_rel), "relax_budget_capped": n_cap,
             "relax_capped_E_rel_median": med(capped_rel), "relax_capped_E_rel_max": mx(capped_rel),
             "eig_abs_max": mx(eig), "eig_abs_median": med(eig), "saddle_E_rel_max": mx(sad),
