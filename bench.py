@@ -176,7 +176,8 @@ def cpu_baseline(po, N, phi0, budget_s=20.0, check_fields=None):
 
 SERVE_WORKLOAD = ("reduced serving stream: %d Poisson requests at %.0f/s, 60%% relax (N 64/128/256, to "
                   "|grad F| 1e-6 or 2000 steps), 25%% 4-mode LOBPCG (N 64/128), 15%% SPM 100 iterations (N 128); "
-                  "the 500-request stream with relaxations to the minimum is profiles/serving_500.md")
+                  "configuration 5 as written (500 requests at 8/s, relaxations to |grad F| 1e-8 within 100000 "
+                  "steps, 10 LOBPCG calls, Spring_Pairing(30000, 1e-9)) is served by serve.py --as-written")
 
 
 def serve_stream(n, rate):
