@@ -113,3 +113,24 @@ def test_worker_requests_identical_under_concurrent_load(serve, stream, spm):
             assert a["eigenvalues"] == m["eigenvalues"], (r["id"], a, m)
         else:
             assert np.isfinite(m["E"]) and a["E"] == m["E"] and a["res"] == m["res"], (r["id"], a, m)
+
+
+def test_as_written_relaxations_hand_over_and_converge(serve):
+    """Configuration 5 as written: a mixed relaxation runs FP32 steps until
+    ||grad F|| <= switch_tol (1e-6), then FP64 steps to the request's 1e-8, and
+    ends where the FP64-only run ends (both stop at the tolerance: energies
+    within ||grad F||^2 / lambda_min, 1e-12 absolute here)."""
+    kw = dict(serve.AS_WRITTEN, relax_iters=20000)
+    reqs = serve.make_requests(n=16, rate=400.0, seed=5, relax_sizes=(32,), modes_sizes=(32,), spm_size=32, **kw)
+    relax = [r for r in reqs if r["kind"] == "relax"]
+    assert relax and all(r["switch_tol"] == 1e-6 and r["tol"] == 1e-8 for r in relax)
+    mix, st = serve.serve(relax, {}, mixed=True, chunk=64, workers=1)
+    assert st["errors"] == 0
+    both = 0
+    for r, m in zip(relax, mix):
+        f = serve.run_relax_single(r)
+        if m["gn"] <= 1e-8 and f["gn"] <= 1e-8:
+            both += 1
+            assert m["iters_fp32"] > 0 and m["iters_fp64"] > 0, m  # the hand-over happened
+            assert abs(m["E"] - f["E"]) <= 1e-12, (r, m, f)
+    assert both >= 2
