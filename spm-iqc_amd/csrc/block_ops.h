@@ -32,7 +32,7 @@ int block_means(spm_ctx *h, const void *const *X, int m, double *out);
 // copy field src -> dst
 int field_copy(spm_ctx *h, void *dst, const void *src);
 int field_zero(spm_ctx *h, void *dst);
-// 16 device doubles of the block scratch that no block operation uses
+// 64 device doubles of the block scratch that no block operation uses
 double *block_small(spm_ctx *h);
 
 // Cholesky factor R of the symmetric n x n S (row-major) with the reference's

@@ -20,6 +20,9 @@ struct spm_ctx {
     // block-operation scratch (Gram partials, coefficients)
     double *bscr = nullptr;
     size_t bscr_bytes = 0;
+    // spectra of the operators applied to several fields in one launch, grown on demand
+    void *specws = nullptr;
+    size_t specws_bytes = 0;
     // gradient-flow steps replayed from a graph captured on `own` (valid while
     // the history buffer it writes stays where it was at capture)
     hipGraphExec_t gf_exec = nullptr;

@@ -239,6 +239,68 @@ struct GridImpl final : GridIface {
         precond_in(c, u, OpResid<T>{alpha, (const T *)v}, w);
     }
 
+    bool hv_many(Ctx &c, const void *phi, const void *v0, void *out0, int n, void *spec, double *means,
+                 double *dots) override
+    {
+        if constexpr (N == 64 || N == 128 || N == 256) {
+            if (two_pass(c) && n >= 1 && n <= BATCH_OPS) {
+                const double sc = 1.0 / (double)c.R;
+                const long fs = c.R;
+                sj_rows<false, true>(c, v0, spec, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0), nullptr,
+                                     ProNone{}, n, fs, c.C);
+                launch_reduce(c, c.slot(0), SJT::R_BLOCKS, n, 0u, sc, means);
+                sj_cols<true, true>(c, spec, spec, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr, ProNone{}, n, c.C);
+                if (dots) {
+                    OpHvOutM<T, true> out{};
+                    out.Tau = c.Tau;
+                    out.Gamma = c.Gamma;
+                    out.phi = (const T *)phi;
+                    out.v = (const T *)v0;
+                    out.hv = (T *)out0;
+                    out.fs = fs;
+                    out.means = means;
+                    sj_rows<true, false>(c, spec, nullptr, out, c.slot(1), nullptr, ProNone{}, n, c.C, 0);
+                    launch_reduce(c, c.slot(1), SJT::R_BLOCKS, n, 0u, sc, dots);
+                } else {
+                    OpHvOutM<T, false> out{};
+                    out.Tau = c.Tau;
+                    out.Gamma = c.Gamma;
+                    out.phi = (const T *)phi;
+                    out.v = (const T *)v0;
+                    out.hv = (T *)out0;
+                    out.fs = fs;
+                    out.means = means;
+                    sj_rows<true, false>(c, spec, nullptr, out, nullptr, nullptr, ProNone{}, n, c.C, 0);
+                }
+                return true;
+            }
+        }
+        return false;
+    }
+
+    bool precond_resid_many(Ctx &c, const void *u0, const void *v0, const double *alphas, void *w0, int n,
+                            void *spec) override
+    {
+        if constexpr (N == 64 || N == 128 || N == 256) {
+            if (two_pass(c) && n >= 1 && n <= BATCH_OPS) {
+                const long fs = c.R;
+                OpResidM<T> in{};
+                in.v = (const T *)v0;
+                in.fs = fs;
+                in.alphas = alphas;
+                OpStoreM<T> out{};
+                out.out = (T *)w0;
+                out.fs = fs;
+                sj_rows<false, true>(c, u0, spec, in, nullptr, nullptr, ProNone{}, n, fs, c.C);
+                sj_cols<true, true>(c, spec, spec, SOpMulTab<T, N>{1.0 / (double)c.R, c.Itab}, nullptr, nullptr,
+                                    ProNone{}, n, c.C);
+                sj_rows<true, false>(c, spec, nullptr, out, nullptr, nullptr, ProNone{}, n, c.C, 0);
+                return true;
+            }
+        }
+        return false;
+    }
+
     void gconv(Ctx &c, const void *in, void *out) override
     {
         if constexpr (N == 64 || N == 128 || N == 256) {

@@ -308,6 +308,42 @@ template <typename T> struct OpResid {
     __device__ __forceinline__ T operator()(T x, long g, double *acc) const { return (*this)(x, v[g], g, acc); }
 };
 
+// The same three operators over a block of fields processed as batch members
+// (blockIdx.y = m): member m reads and writes the fields at m * fs elements
+// past the first, and its own mean / alpha at index m of a device array.
+template <typename T, bool DOT = false> struct OpHvOutM : OpHvOut<T, DOT> {
+    long fs;
+    const double *means;
+    __device__ __forceinline__ OpHvOutM bind(int m) const
+    {
+        OpHvOutM o = *this;
+        o.v = this->v + (long)m * fs;
+        o.hv = this->hv + (long)m * fs;
+        o.mean_cv = means + m;
+        return o;
+    }
+};
+template <typename T> struct OpResidM : OpResid<T> {
+    long fs;
+    const double *alphas;
+    __device__ __forceinline__ OpResidM bind(int m) const
+    {
+        OpResidM o = *this;
+        o.v = this->v + (long)m * fs;
+        o.alpha = alphas[m];
+        return o;
+    }
+};
+template <typename T> struct OpStoreM : OpStore<T> {
+    long fs;
+    __device__ __forceinline__ OpStoreM bind(int m) const
+    {
+        OpStoreM o = *this;
+        o.out = this->out + (long)m * fs;
+        return o;
+    }
+};
+
 // Natural_Forces / MinMax_Forces input (SPM.cpp:53-86 with beta = 1): the
 // three transforms FFT(x) + FFT(nln - mean) - FFT(coef <f, v> v) of the
 // reference are one transform of y = x + nln(x) - coef s v, s = <f, v> (a

@@ -128,6 +128,77 @@ struct LobBlock {
 
 static std::vector<const void *> cv(const std::vector<void *> &a) { return std::vector<const void *>(a.begin(), a.end()); }
 
+// fields a[0..n) one after another (R values apart): a block the batched
+// operators can take as members
+static bool contiguous(const std::vector<void *> &a, int n, const Ctx &c)
+{
+    for (int i = 1; i < n; ++i)
+        if ((char *)a[i] != (char *)a[0] + (size_t)i * c.R * c.esz) return false;
+    return true;
+}
+
+// Fields per launch of the batched operators: BATCH_OPS where a field's
+// spectrum is at most 64 MB (N <= 128), where the operator passes are short and
+// their launches cost a large share of an iteration; one at 256^3, where the
+// passes are long and a batch gained nothing (LOBPCG iteration, block 8, two
+// passes: 64^3 1.04 -> 0.50 ms FP64, 128^3 2.35 -> 1.75 ms; 256^3 12.78 against
+// 12.93 ms FP64 and 8.95 against 8.57 ms FP32).
+static int batch_chunk(const Ctx &c)
+{
+    const double spec = (double)c.C * 2 * c.esz;
+    return spec <= 64e6 ? (int)GridIface::BATCH_OPS : 1;
+}
+
+static int ensure_specws(spm_ctx *h, int n)
+{
+    const size_t bytes = (size_t)n * h->c.C * 2 * h->c.esz;
+    if (h->specws_bytes >= bytes) return SPM_OK;
+    if (h->specws) {
+        HIPCHK(hipStreamSynchronize(h->c.stream));
+        HIPCHK(hipFree(h->specws));
+        h->dev_bytes -= h->specws_bytes;
+        h->specws = nullptr;
+        h->specws_bytes = 0;
+    }
+    RCHK(spm_dev_alloc(h, &h->specws, bytes));
+    h->specws_bytes = bytes;
+    return SPM_OK;
+}
+
+// out_i = H in_i (hv_cam, LOBPCG.cpp:113-166) for the fields with keep[i] (keep
+// NULL: all), the others' outputs left alone; with dots, dots[i] = <in_i, out_i>
+// (device).  Contiguous blocks with every field kept go through the batched
+// operators, batch_chunk fields per launch of each pass (the same kernels and
+// sums per field as field by field, so the same values); anything else field by
+// field.
+static int hv_block(spm_ctx *h, const void *phi, const std::vector<void *> &in, const std::vector<void *> &out,
+                    const bool *keep, double *dots)
+{
+    Ctx &c = h->c;
+    const int n = (int)in.size();
+    bool all = true;
+    for (int i = 0; keep && i < n; ++i) all = all && keep[i];
+    const int chunk = std::min(n, batch_chunk(c));
+    if (all && chunk > 1 && contiguous(in, n, c) && contiguous(out, n, c)) {
+        double *means = block_small(h);
+        if (!means) return spm_fail(SPM_ENOMEM, "lobpcg: block scratch");
+        means += 16;
+        RCHK(ensure_specws(h, chunk));
+        bool done = true;
+        for (int i0 = 0; i0 < n && done; i0 += chunk)
+            done = c.g->hv_many(c, phi, in[i0], out[i0], std::min(chunk, n - i0), h->specws, means,
+                                dots ? dots + i0 : nullptr);
+        if (done) return spm_launch_status("lobpcg hv");
+        // no batched form (then nothing was launched): field by field below
+    }
+    for (int i = 0; i < n; ++i) {
+        if (keep && !keep[i]) continue;
+        if (dots) c.g->hv_dot(c, phi, in[i], out[i], dots + i);
+        else c.g->hv(c, phi, in[i], out[i]);
+    }
+    return spm_launch_status("lobpcg hv");
+}
+
 // V = the k fields at Vout, U..T the first 6k fields of the workspace (zeroed)
 static int lob_block(spm_ctx *h, void *Vout, int k, size_t extra_fields, LobBlock &b)
 {
@@ -261,14 +332,23 @@ static int lobpcg_sweep(spm_ctx *h, const void *phi, LobBlock &b, double *ritz)
     // U = H V, alpha (:281-284)
     double *adev = block_small(h);
     if (!adev) return spm_fail(SPM_ENOMEM, "lobpcg: block scratch");
-    for (int i = 0; i < k; ++i) c.g->hv_dot(c, phi, V[i], U[i], adev + i);
-    RCHK(spm_launch_status("lobpcg hv"));
-    std::vector<double> alpha(k);
-    HIPCHK(hipMemcpyAsync(h->h_pin, adev, sizeof(double) * k, hipMemcpyDeviceToHost, c.stream));
-    HIPCHK(hipStreamSynchronize(c.stream));
-    std::memcpy(alpha.data(), h->h_pin, sizeof(double) * k);
-    // W = precond(U - alpha V) (:284-300)
-    for (int i = 0; i < k; ++i) c.g->precond_resid(c, U[i], V[i], alpha[i], W[i]);
+    RCHK(hv_block(h, phi, V, U, nullptr, adev));
+    // W = precond(U - alpha V) (:284-300): batched with alpha on the device, or field by field
+    const int chunk = std::min(k, batch_chunk(c));
+    bool batched = chunk > 1 && contiguous(V, k, c) && contiguous(U, k, c) && contiguous(W, k, c);
+    if (batched) {
+        RCHK(ensure_specws(h, chunk));
+        for (int i0 = 0; i0 < k && batched; i0 += chunk)
+            batched = c.g->precond_resid_many(c, U[i0], V[i0], adev + i0, W[i0], std::min(chunk, k - i0),
+                                              h->specws);
+    }
+    if (!batched) {
+        std::vector<double> alpha(k);
+        HIPCHK(hipMemcpyAsync(h->h_pin, adev, sizeof(double) * k, hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        std::memcpy(alpha.data(), h->h_pin, sizeof(double) * k);
+        for (int i = 0; i < k; ++i) c.g->precond_resid(c, U[i], V[i], alpha[i], W[i]);
+    }
     RCHK(spm_launch_status("lobpcg precond"));
     // G = [V, W, Vp]^T [V, W, Vp]
     const int n3 = 3 * k;
@@ -399,11 +479,8 @@ static int lobpcg_rayleigh_ritz(spm_ctx *h, const void *phi, LobBlock &b, double
     const int k = (int)b.V.size();
     std::vector<void *> &V = b.V, &U = b.U, &W = b.W, &Y = b.Y, &Vp = b.Vp, &Up = b.Up, &T = b.T;
     bool *keepV = b.keepV(), *keepW = b.keepW(), *keepP = b.keepP();
-    for (int i = 0; i < k; ++i)
-        if (keepW[i]) c.g->hv(c, phi, W[i], Y[i]);
-    for (int i = 0; i < k; ++i)
-        if (keepP[i]) c.g->hv(c, phi, Vp[i], Up[i]);
-    RCHK(spm_launch_status("lobpcg hv"));
+    RCHK(hv_block(h, phi, W, Y, keepW, nullptr));
+    RCHK(hv_block(h, phi, Vp, Up, keepP, nullptr));
     std::vector<const void *> X, YY;
     for (int i = 0; i < k; ++i) {
         X.push_back(V[i]);
@@ -490,8 +567,7 @@ static int lobpcg_sweep_ref(spm_ctx *h, const void *phi, LobBlock &b, double *ri
     std::vector<void *> &V = b.V, &U = b.U, &W = b.W, &Y = b.Y, &Vp = b.Vp, &Up = b.Up, &T = b.T;
     bool *keepV = b.keepV(), *keepW = b.keepW(), *keepP = b.keepP();
     if (!from_w) {
-        for (int i = 0; i < k; ++i) c.g->hv(c, phi, V[i], U[i]);
-        RCHK(spm_launch_status("lobpcg hv"));
+        RCHK(hv_block(h, phi, V, U, nullptr, nullptr));
         std::vector<double> VU(k * k);
         RCHK(block_gram(h, cv(V).data(), k, cv(U).data(), k, VU.data()));
         {   // W = U - alpha V  (FuncsLinear2Cplx, :284)
@@ -515,10 +591,9 @@ static int lobpcg_sweep_ref(spm_ctx *h, const void *phi, LobBlock &b, double *ri
         RCHK(block_project(h, W.data(), k, cv(V).data(), k));
         RCHK(block_ortho(h, W.data(), k, b.V0.data(), k0, 1e-8, keepW));
     }
-    for (int i = 0; i < k; ++i) {
-        if (keepW[i]) c.g->hv(c, phi, W[i], Y[i]);
-        else RCHK(field_zero(h, Y[i]));
-    }
+    RCHK(hv_block(h, phi, W, Y, keepW, nullptr));
+    for (int i = 0; i < k; ++i)
+        if (!keepW[i]) RCHK(field_zero(h, Y[i]));
     if (k0 == 0) {
         std::vector<const void *> B(2 * k);
         for (int i = 0; i < k; ++i) {
@@ -531,11 +606,9 @@ static int lobpcg_sweep_ref(spm_ctx *h, const void *phi, LobBlock &b, double *ri
         RCHK(block_project(h, Vp.data(), k, cv(W).data(), k));
         RCHK(block_ortho(h, Vp.data(), k, b.V0.data(), k0, 1e-8, keepP));
     }
-    for (int i = 0; i < k; ++i) {
-        if (keepP[i]) c.g->hv(c, phi, Vp[i], Up[i]);
-        else RCHK(field_zero(h, Up[i]));
-    }
-    RCHK(spm_launch_status("lobpcg hv"));
+    RCHK(hv_block(h, phi, Vp, Up, keepP, nullptr));
+    for (int i = 0; i < k; ++i)
+        if (!keepP[i]) RCHK(field_zero(h, Up[i]));
     // Rayleigh-Ritz on the kept basis (get_Merge / get_mat, :332-362)
     std::vector<const void *> UU, YY;
     for (int i = 0; i < k; ++i) {

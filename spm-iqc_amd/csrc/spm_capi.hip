@@ -203,7 +203,7 @@ static void free_all(spm_ctx *h)
     if (h->gf_exec) (void)hipGraphExecDestroy(h->gf_exec);
     if (h->ev_in) (void)hipEventDestroy(h->ev_in);
     if (h->ev_out) (void)hipEventDestroy(h->ev_out);
-    for (void *p : {h->ws, (void *)h->bscr})
+    for (void *p : {h->ws, (void *)h->bscr, h->specws})
         if (p) (void)hipFree(p);
     if (h->own) (void)hipStreamDestroy(h->own);
     delete c.g;

@@ -95,6 +95,16 @@ struct GridIface {
     virtual void precond(Ctx &c, void *w) = 0;
     // w = precond(u - alpha v)  (w may not alias u or v)
     virtual void precond_resid(Ctx &c, const void *u, const void *v, double alpha, void *w) = 0;
+    // hv_cam / precond_camnew of n fields at once (n <= BATCH_OPS fields at a stride of
+    // R values from v0, u0, ...), each a member of one launch per pass; spec: n
+    // spectra of scratch; means: n device doubles; dots (may be NULL): <v_i, H v_i>;
+    // alphas: n device doubles.  False where the operators have no batched form
+    // (the axis-by-axis passes): the caller then goes field by field.
+    static constexpr int BATCH_OPS = 8;
+    virtual bool hv_many(Ctx &c, const void *phi, const void *v0, void *out0, int n, void *spec, double *means,
+                         double *dots) = 0;
+    virtual bool precond_resid_many(Ctx &c, const void *u0, const void *v0, const double *alphas, void *w0, int n,
+                                    void *spec) = 0;
     // out = IFFT(G FFT(in) / N^3) with the DC set to zero (the non-local part of ngrad_cam)
     virtual void gconv(Ctx &c, const void *in, void *out) = 0;
     // x <- IFFT(FFT(x + nln(x) - coef * (*s) * v) / N^3 / (1 + G)), DC = 0
