@@ -129,9 +129,14 @@ def footprint(r):
         return 2 * N * N * (N // 2 + 1 + 16) * 2 * b + CTX_SCRATCH
 
     if r["kind"] == "modes":
-        k = 8  # workspace U, W, Y, Vp, Up, T (6k fields), the block V (FP64 outside), phi
-        fp32 = ctx(4) + (6 * k + 2) * R * 4 + k * R * 8
-        fp64 = ctx(8) + (6 * k + 1) * R * 8 + k * R * 8
+        k = 8  # workspace U, W, Y, Vp, Up, T (6k fields), the block V (FP64 outside), phi; at N <= 128
+        # the block's operators run as batch members with k spectra of their own
+
+        def spectra(b):
+            return k * N * N * (N // 2 + 16) * 2 * b if N <= 128 else 0
+
+        fp32 = ctx(4) + (6 * k + 2) * R * 4 + k * R * 8 + spectra(4)
+        fp64 = ctx(8) + (6 * k + 1) * R * 8 + k * R * 8 + spectra(8)
         return max(fp32, fp64)
     # saddle: two replicas, d and the kept spectrum (workspace), ngrad scratch; the FP64
     # replicas made from the FP32 ones, and phi, v1
