@@ -43,8 +43,10 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
     for (int tb = 0; tb < TB; ++tb)
         yb[tb] = (!SAME && tb * 16 + row < mb) ? (const T *)Y.p[tb * 16 + row] : nullptr;
     const long nchunk = R / 16;
-    // lane data of chunk q: 4 consecutive points r0 + 4 (l / 16) .. +3 of its row's field
-    auto load = [&](long q, double (&a)[TA][4], double (&b)[TB][4]) {
+    // lane data of chunk q: 4 consecutive points r0 + 4 (l / 16) .. +3 of its row's field,
+    // in the field precision (FP32 fields: half the registers in flight), widened to
+    // FP64 (exactly) where an MFMA takes it
+    auto load = [&](long q, T (&a)[TA][4], T (&b)[TB][4]) {
         const long r0 = q * 16 + 4 * kq;
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta) {
@@ -57,7 +59,7 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
                     a[ta][0] = u.x; a[ta][1] = u.y; a[ta][2] = u.z; a[ta][3] = u.w;
                 }
             } else {
-                a[ta][0] = a[ta][1] = a[ta][2] = a[ta][3] = 0.0;
+                a[ta][0] = a[ta][1] = a[ta][2] = a[ta][3] = (T)0;
             }
         }
 #pragma unroll
@@ -72,17 +74,17 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
                     b[tb][0] = u.x; b[tb][1] = u.y; b[tb][2] = u.z; b[tb][3] = u.w;
                 }
             } else {
-                b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = 0.0;
+                b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = (T)0;
             }
         }
     };
     // one chunk ahead: the next chunk's loads are in flight during this chunk's MFMAs
     const long step = (long)gridDim.x * 4;
     long q = (long)blockIdx.x * 4 + wid;
-    double a[TA][4], b[TB][4];
+    T a[TA][4], b[TB][4];
     if (q < nchunk) load(q, a, b);
     for (; q < nchunk; q += step) {
-        double an[TA][4], bn[TB][4];
+        T an[TA][4], bn[TB][4];
         if (q + step < nchunk) load(q + step, an, bn);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -90,8 +92,8 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
             for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
                 for (int tb = 0; tb < TB; ++tb)
-                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], SAME ? a[tb][s] : b[tb][s],
-                                                                       acc[ta][tb], 0, 0, 0);
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        (double)a[ta][s], (double)(SAME ? a[tb][s] : b[tb][s]), acc[ta][tb], 0, 0, 0);
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
@@ -247,7 +249,9 @@ __global__ void __launch_bounds__(256) k_gram_xy(FieldPtrs X, int mx, FieldPtrs 
         reuse[tb] = tb < TA && cc < mx;
         yb[tb] = (cc >= mx && cc - mx < my) ? (const T *)Y.p[cc - mx] : nullptr;
     }
-    auto ld4 = [&](const T *f, long r0, double (&v)[4]) {
+    // Loads keep the field precision (FP32 fields: half the registers in flight);
+    // each value is widened to FP64 (exactly) where an MFMA takes it.
+    auto ld4 = [&](const T *f, long r0, T (&v)[4]) {
         if constexpr (sizeof(T) == 8) {
             const double2 u = *(const double2 *)(f + r0), w = *(const double2 *)(f + r0 + 2);
             v[0] = u.x; v[1] = u.y; v[2] = w.x; v[3] = w.y;
@@ -257,34 +261,34 @@ __global__ void __launch_bounds__(256) k_gram_xy(FieldPtrs X, int mx, FieldPtrs 
         }
     };
     const long nchunk = R / 16;
-    auto load = [&](long q, double (&a)[TA][4], double (&b)[TB][4]) {
+    auto load = [&](long q, T (&a)[TA][4], T (&b)[TB][4]) {
         const long r0 = q * 16 + 4 * kq;
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta) {
             if (xa[ta]) ld4(xa[ta], r0, a[ta]);
-            else a[ta][0] = a[ta][1] = a[ta][2] = a[ta][3] = one[ta] ? 1.0 : 0.0;
+            else a[ta][0] = a[ta][1] = a[ta][2] = a[ta][3] = one[ta] ? (T)1 : (T)0;
         }
 #pragma unroll
         for (int tb = 0; tb < TB; ++tb) {
             if (yb[tb]) ld4(yb[tb], r0, b[tb]);
-            else b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = 0.0;
+            else b[tb][0] = b[tb][1] = b[tb][2] = b[tb][3] = (T)0;
         }
     };
     const long step = (long)gridDim.x * 4;
     long q = (long)blockIdx.x * 4 + wid;
-    double a[TA][4], b[TB][4];
+    T a[TA][4], b[TB][4];
     if (q < nchunk) load(q, a, b);
     for (; q < nchunk; q += step) {
-        double an[TA][4], bn[TB][4];
+        T an[TA][4], bn[TB][4];
         if (q + step < nchunk) load(q + step, an, bn);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int tb = 0; tb < TB; ++tb) {
-                const double bv = (tb < TA && reuse[tb]) ? a[tb < TA ? tb : 0][s] : b[tb][s];
+                const double bv = (double)((tb < TA && reuse[tb]) ? a[tb < TA ? tb : 0][s] : b[tb][s]);
 #pragma unroll
                 for (int ta = 0; ta < TA; ++ta)
-                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta][s], bv, acc[ta][tb], 0, 0, 0);
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[ta][s], bv, acc[ta][tb], 0, 0, 0);
             }
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta)
