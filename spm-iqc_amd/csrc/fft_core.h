@@ -114,8 +114,12 @@ template <int A, int B, int DIR, typename T> __device__ __forceinline__ void dft
         for (int k1 = 0; k1 < A; ++k1) {
             const int e = (n2 * k1) % R;
             if (e != 0) {
-                const C2<T> w = {kcos<T, R>(e), DIR < 0 ? -ksin<T, R>(e) : ksin<T, R>(e)};
-                y[n2][k1] = cmul(y[n2][k1], w);
+                // constants after unrolling; +-i and -1 are taken exactly, without
+                // the products by zero a general complex product would keep
+                const T cr = kcos<T, R>(e), si = DIR < 0 ? -ksin<T, R>(e) : ksin<T, R>(e);
+                if (cr == (T)0) y[n2][k1] = si > (T)0 ? mul_i<+1>(y[n2][k1]) : mul_i<-1>(y[n2][k1]);
+                else if (si == (T)0) y[n2][k1] = C2<T>{-y[n2][k1].x, -y[n2][k1].y};
+                else y[n2][k1] = cmul(y[n2][k1], C2<T>{cr, si});
             }
         }
     }

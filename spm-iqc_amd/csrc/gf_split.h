@@ -246,20 +246,17 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
             hold[h] = c[h].y;
         }
         __syncthreads();
-        if (tid < S1) { // the kz = N/2 columns: only their real parts enter the c2r
-            const int ln = S1 * lo + tid;
+        if (tid < DH * S1) { // the kz = N/2 columns, one output per lane: only their
+                             // real parts enter the c2r
+            // x[h] = Re sum_k conj(W^(k (DL h + ln))) y[k], W = exp(-2 pi i / N)
+            const int cn = tid / DH, hn = tid % DH, ln = S1 * lo + cn;
+            T sx = 0;
 #pragma unroll
-            for (int h = 0; h < DH; ++h) {
-                c[h] = nyin[h * S1 + tid];
-                if (h > 0) {
-                    C2<T> w = stw[twpad(ln * h)];
-                    w.y = -w.y;
-                    c[h] = cmul(c[h], w);
-                }
+            for (int k = 0; k < DH; ++k) {
+                const C2<T> w = stw[twpad((k * (DL * hn + ln)) % N)], y = nyin[k * S1 + cn];
+                sx += w.x * y.x + w.y * y.y;
             }
-            dft<DH, +1>(c);
-#pragma unroll
-            for (int h = 0; h < DH; ++h) nyq[h * S1 + tid] = c[h].x;
+            nyq[hn * S1 + cn] = sx;
         }
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -349,16 +346,17 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         // Z goes through LDS as real parts, then imaginary parts; the kz = N/2
         // entries (real) go to nyq.
         static_assert(S1 == 2, "pair h = rows (h, 2 lo), (h, 2 lo + 1)");
-        T zr[DH], zs[DH];
+        // With Z = Zr + i Zi at kz and Zs + i Zj at N - kz: Xa = (Zr + Zs, Zi - Zj) / 2
+        // and Xb = (Zi + Zj, Zs - Zr) / 2; the real-part term u waits for the second
+        // exchange (sg = +1 for l even, -1 for l odd)
+        const T sg = li == 0 ? (T)1 : (T)-1;
+        T u[DH];
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < E; ++m) my[t + TPP * m] = v[m].x;
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < DH; ++h) {
-            zr[h] = lds[h * RSTR + kz];
-            zs[h] = lds[h * RSTR + (N - kz) % N];
-        }
+        for (int h = 0; h < DH; ++h) u[h] = fma(sg, lds[h * RSTR + kz], lds[h * RSTR + (N - kz) % N]);
         if (tid < DH) nyq[tid * S1] = lds[tid * RSTR + N / 2]; // Xa[N/2] = Re Z[N/2]
         __syncthreads();
 #pragma unroll
@@ -369,7 +367,8 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
 #pragma unroll
             for (int h = 0; h < DH; ++h) {
                 const T zi = lds[h * RSTR + kz], zj = lds[h * RSTR + (N - kz) % N];
-                c[h] = li == 0 ? C2<T>{hx * (zr[h] + zs[h]), hx * (zi - zj)} : C2<T>{hx * (zi + zj), hx * (zs[h] - zr[h])};
+                const T w = fma(-sg, zj, zi); // l even: Zi - Zj; l odd: Zi + Zj
+                c[h] = li == 0 ? C2<T>{hx * u[h], hx * w} : C2<T>{hx * w, hx * u[h]};
             }
         }
         if (tid >= DH && tid < 2 * DH) nyq[(tid - DH) * S1 + 1] = lds[(tid - DH) * RSTR + N / 2]; // Xb[N/2] = Im Z[N/2]
@@ -384,13 +383,18 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         stage_a(l);
 #pragma unroll
         for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, ko, l, ki)] = c[h];
-        if (tid < S1) { // kz = N/2 columns (real after the r2c)
-            const int ln = S1 * lo + tid;
+        if (tid < DH * S1) { // kz = N/2 columns (real after the r2c), one output per lane:
+                             // X[k1] = sum_h W^(k1 (DL h + ln)) x[h]
+            const int cn = tid / DH, k1 = tid % DH, ln = S1 * lo + cn;
+            C2<T> sx = {(T)0, (T)0};
 #pragma unroll
-            for (int h = 0; h < DH; ++h) c[h] = C2<T>{nyq[h * S1 + tid], (T)0};
-            stage_a(ln);
-#pragma unroll
-            for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, (N / 2) / KT, ln, (N / 2) % KT)] = c[h];
+            for (int h = 0; h < DH; ++h) {
+                const C2<T> w = stw[twpad((k1 * (DL * h + ln)) % N)];
+                const T x = nyq[h * S1 + cn];
+                sx.x += w.x * x;
+                sx.y += w.y * x;
+            }
+            Wo[S::wof(i, k1, (N / 2) / KT, ln, (N / 2) % KT)] = sx;
         }
     }
     write_partials<Op::NRED, Op::MAXM>(acc, red, part);
