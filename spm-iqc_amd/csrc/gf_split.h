@@ -246,17 +246,33 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
             hold[h] = c[h].y;
         }
         __syncthreads();
-        if (tid < DH * S1) { // the kz = N/2 columns, one output per lane: only their
-                             // real parts enter the c2r
+        // the kz = N/2 columns: only their real parts enter the c2r
+        if constexpr (DH >= 8) { // one output per lane,
             // x[h] = Re sum_k conj(W^(k (DL h + ln))) y[k], W = exp(-2 pi i / N)
-            const int cn = tid / DH, hn = tid % DH, ln = S1 * lo + cn;
-            T sx = 0;
+            if (tid < DH * S1) {
+                const int cn = tid / DH, hn = tid % DH, ln = S1 * lo + cn;
+                T sx = 0;
 #pragma unroll
-            for (int k = 0; k < DH; ++k) {
-                const C2<T> w = stw[twpad((k * (DL * hn + ln)) % N)], y = nyin[k * S1 + cn];
-                sx += w.x * y.x + w.y * y.y;
+                for (int k = 0; k < DH; ++k) {
+                    const C2<T> w = stw[twpad((k * (DL * hn + ln)) % N)], y = nyin[k * S1 + cn];
+                    sx += w.x * y.x + w.y * y.y;
+                }
+                nyq[hn * S1 + cn] = sx;
             }
-            nyq[hn * S1 + cn] = sx;
+        } else if (tid < S1) { // a 4-point transform per column (N = 64: one wave per block)
+            const int ln = S1 * lo + tid;
+#pragma unroll
+            for (int h = 0; h < DH; ++h) {
+                c[h] = nyin[h * S1 + tid];
+                if (h > 0) {
+                    C2<T> w = stw[twpad(ln * h)];
+                    w.y = -w.y;
+                    c[h] = cmul(c[h], w);
+                }
+            }
+            dft<DH, +1>(c);
+#pragma unroll
+            for (int h = 0; h < DH; ++h) nyq[h * S1 + tid] = c[h].x;
         }
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -383,18 +399,27 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         stage_a(l);
 #pragma unroll
         for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, ko, l, ki)] = c[h];
-        if (tid < DH * S1) { // kz = N/2 columns (real after the r2c), one output per lane:
-                             // X[k1] = sum_h W^(k1 (DL h + ln)) x[h]
-            const int cn = tid / DH, k1 = tid % DH, ln = S1 * lo + cn;
-            C2<T> sx = {(T)0, (T)0};
+        // kz = N/2 columns (real after the r2c)
+        if constexpr (DH >= 8) { // one output per lane, X[k1] = sum_h W^(k1 (DL h + ln)) x[h]
+            if (tid < DH * S1) {
+                const int cn = tid / DH, k1 = tid % DH, ln = S1 * lo + cn;
+                C2<T> sx = {(T)0, (T)0};
 #pragma unroll
-            for (int h = 0; h < DH; ++h) {
-                const C2<T> w = stw[twpad((k1 * (DL * h + ln)) % N)];
-                const T x = nyq[h * S1 + cn];
-                sx.x += w.x * x;
-                sx.y += w.y * x;
+                for (int h = 0; h < DH; ++h) {
+                    const C2<T> w = stw[twpad((k1 * (DL * h + ln)) % N)];
+                    const T x = nyq[h * S1 + cn];
+                    sx.x += w.x * x;
+                    sx.y += w.y * x;
+                }
+                Wo[S::wof(i, k1, (N / 2) / KT, ln, (N / 2) % KT)] = sx;
             }
-            Wo[S::wof(i, k1, (N / 2) / KT, ln, (N / 2) % KT)] = sx;
+        } else if (tid < S1) {
+            const int ln = S1 * lo + tid;
+#pragma unroll
+            for (int h = 0; h < DH; ++h) c[h] = C2<T>{nyq[h * S1 + tid], (T)0};
+            stage_a(ln);
+#pragma unroll
+            for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, (N / 2) / KT, ln, (N / 2) % KT)] = c[h];
         }
     }
     write_partials<Op::NRED, Op::MAXM>(acc, red, part);
