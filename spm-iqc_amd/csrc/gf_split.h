@@ -526,6 +526,9 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
                 for (int q = q0; q < q0 + 4; ++q) op(v[q], pre[SOp::PRELOAD ? q : 0], ub + q * LSTR, ia, h + DH * q, kz, acc);
             }
         }
+        if constexpr (defers_store<SOp>::value)
+#pragma unroll
+            for (int q = 0; q < DL; ++q) op.store_u(ub + q * LSTR, v[q]);
     }
     if constexpr (INV) {
         dft<16, +1>(v);

@@ -87,6 +87,14 @@ template <class Op> struct op_chunk<Op, std::void_t<decltype(Op::CHUNK)>> : std:
 template <class Op, class = void> struct has_aux : std::false_type {};
 template <class Op> struct has_aux<Op, std::void_t<decltype(std::declval<const Op &>().aux())>> : std::true_type {};
 
+// A spectral operator that updates a state array in place may leave the store
+// of the new state to the pass (DEFER_STORE: the new state is the value it
+// returns), which stores it after the whole update: on gfx9 loads and stores
+// share one counter, so a store between two elements' loads makes the second
+// element wait for the store to complete.
+template <class S, class = void> struct defers_store : std::false_type {};
+template <class S> struct defers_store<S, std::void_t<decltype(S::DEFER_STORE)>> : std::bool_constant<S::DEFER_STORE> {};
+
 // ======================================================================
 // Real-space operators applied to each real value inside the row pass.
 // op(x, g, acc): x = value at real-field index g; returns the value the pass
@@ -513,6 +521,7 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
     static constexpr int NRED = 3;
     static constexpr unsigned MAXM = 4u; // acc[2] is a max
     static constexpr bool PRELOAD = PRE, FINALIZE = false;
+    static constexpr bool DEFER_STORE = true; // the pass stores the new U (= the value returned in v)
     double sc, dt;
     const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
@@ -567,7 +576,6 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
             }
             acc[0] += (double)(w * g * (ur * ur + ui * ui));
             acc[2] = fmax(acc[2], (double)fabsf(nr - ur));
-            store_u(s, C2<T>{nr, ni});
             v = C2<T>{nr, ni};
             return;
         }
@@ -585,7 +593,6 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
         acc[0] += w * g * (ur * ur + ui * ui);
         const C2<T> un = {(T)nr, (T)ni};
         acc[2] = fmax(acc[2], fabs((double)un.x - ur));
-        store_u(s, un);
         v = un;
     }
 };
@@ -913,6 +920,9 @@ __global__ void __launch_bounds__((Geo<T, N, TPP_>::COL_THREADS))
             const int j = iaxis ? (int)(col / NHP) : e;
             op(v[m], pre[SOp::PRELOAD ? m : 0], off + (long)e * pstride, i, j, kz, acc);
         }
+        if constexpr (defers_store<SOp>::value)
+#pragma unroll
+            for (int m = 0; m < E; ++m) op.store_u(off + (long)(t + TPP * m) * pstride, v[m]);
     }
     if constexpr (INV) {
         if constexpr (G::XSPLIT)
