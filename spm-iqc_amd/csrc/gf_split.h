@@ -522,8 +522,23 @@ __global__ void __launch_bounds__((SJ<T, N>::C_THREADS))
 #pragma unroll
         for (int q0 = 0; q0 < DL; q0 += 4) {
             if (blockIdx.x < gridDim.x + q0) {
+                if constexpr (gathers_group<SOp>::value && N <= 128) { // the group's loads first, then its updates
+                    // (at 256^3 the per-element order is faster)
+                    C2<T> ug[4], gg[4];
+                    int n2[4];
 #pragma unroll
-                for (int q = q0; q < q0 + 4; ++q) op(v[q], pre[SOp::PRELOAD ? q : 0], ub + q * LSTR, ia, h + DH * q, kz, acc);
+                    for (int q = 0; q < 4; ++q) {
+                        n2[q] = SOp::n2of(ia, h + DH * (q0 + q), kz);
+                        gg[q] = op.load_gd(n2[q]);
+                        ug[q] = op.load_pre(ub + (q0 + q) * LSTR);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) op.update(v[q0 + q], ug[q], gg[q], n2[q], kz, acc);
+                } else {
+#pragma unroll
+                    for (int q = q0; q < q0 + 4; ++q)
+                        op(v[q], pre[SOp::PRELOAD ? q : 0], ub + q * LSTR, ia, h + DH * q, kz, acc);
+                }
             }
         }
         if constexpr (defers_store<SOp>::value)

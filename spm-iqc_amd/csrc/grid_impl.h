@@ -484,16 +484,23 @@ struct GridImpl final : GridIface {
             if (c.gf2) {
                 using S = SJT;
                 const ProSJDensity pd{c.slot(2), S::R_BLOCKS, c.slot(4)};
-                if (c.sjv & SPM_STEP_GATHER_U)
-                    sj_cols<true, true>(
-                        c, c.W, c.W,
-                        SOpGradFlowStep<T, N, false>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
-                        c.slot(3), c.ctl, pd, nm, bs);
-                else
-                    sj_cols<true, true>(
-                        c, c.W, c.W,
-                        SOpGradFlowStep<T, N, true>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
-                        c.slot(3), c.ctl, pd, nm, bs);
+                auto pass2 = [&](auto op) { sj_cols<true, true>(c, c.W, c.W, op, c.slot(3), c.ctl, pd, nm, bs); };
+                const double sc = 1.0 / (double)c.R;
+                const CT *GD = (const CT *)c.GDtab;
+                CT *U = (CT *)c.U;
+                // FP32 takes the hint as a template constant, FP64 as a flag (one kernel)
+                const bool nt32 = sizeof(T) == 4 && nt;
+                if (c.sjv & SPM_STEP_GATHER_U) {
+                    if (nt32)
+                        pass2(SOpGradFlowStep<T, N, false, sizeof(T) == 4>{sc, c.dt, GD, U, bs, nt});
+                    else
+                        pass2(SOpGradFlowStep<T, N, false, false>{sc, c.dt, GD, U, bs, nt});
+                } else {
+                    if (nt32)
+                        pass2(SOpGradFlowStep<T, N, true, sizeof(T) == 4>{sc, c.dt, GD, U, bs, nt});
+                    else
+                        pass2(SOpGradFlowStep<T, N, true, false>{sc, c.dt, GD, U, bs, nt});
+                }
                 sj_rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr, c.prm, 0}, c.slot(2), c.ctl,
                                     ProSJFinalize{c.slot(3), S::C_BLOCKS, c.slot(4), (double)c.R, c.ctl, c.hist,
                                                   3 * c.hist_cap},
@@ -501,9 +508,16 @@ struct GridImpl final : GridIface {
                 return;
             }
         }
-        cols_i<true, true, true>(c, c.W, c.W,
-                                 SOpGradFlowStep<T, N>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab, (CT *)c.U, bs, nt},
-                                 c.slot(3), c.ctl, -1, -1, nm, bs);
+        if (sizeof(T) == 4 && nt)
+            cols_i<true, true, true>(c, c.W, c.W,
+                                     SOpGradFlowStep<T, N, true, sizeof(T) == 4>{1.0 / (double)c.R, c.dt,
+                                                                                 (const CT *)c.GDtab, (CT *)c.U, bs, nt},
+                                     c.slot(3), c.ctl, -1, -1, nm, bs);
+        else
+            cols_i<true, true, true>(c, c.W, c.W,
+                                     SOpGradFlowStep<T, N, true, false>{1.0 / (double)c.R, c.dt, (const CT *)c.GDtab,
+                                                                        (CT *)c.U, bs, nt},
+                                     c.slot(3), c.ctl, -1, -1, nm, bs);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr, c.ctl, -1, -1,
                                   ProGradFlowReduce{c.slot(3), c.nb_i, c.slot(2), c.nb_rows, c.slot(4), kf}, nm, bs);
         rows<true, true>(c, c.W, c.W, OpGradFlowRow<T>{c.Tau, c.Gamma, nullptr, c.prm, 0}, c.slot(2), c.ctl,

@@ -94,6 +94,11 @@ template <class Op> struct has_aux<Op, std::void_t<decltype(std::declval<const O
 // element wait for the store to complete.
 template <class S, class = void> struct defers_store : std::false_type {};
 template <class S> struct defers_store<S, std::void_t<decltype(S::DEFER_STORE)>> : std::bool_constant<S::DEFER_STORE> {};
+// A spectral operator with a gathered state (GATHER_GROUP) offers n2of, load_gd,
+// load_pre and update, so the two-pass column kernel can issue the loads of a
+// group of elements together before it updates them.
+template <class S, class = void> struct gathers_group : std::false_type {};
+template <class S> struct gathers_group<S, std::void_t<decltype(S::GATHER_GROUP)>> : std::bool_constant<S::GATHER_GROUP> {};
 
 // ======================================================================
 // Real-space operators applied to each real value inside the row pass.
@@ -517,19 +522,24 @@ template <typename T, int N> struct SOpEnergy {
 // the forward transform; otherwise it is gathered where it is used (fewer
 // registers held across the transform).  G and D = 1 / (1 + dt G) come from one
 // table entry (one gather, and a multiply instead of two FP64 divides per entry).
-template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
+template <typename T, int N, bool PRE = true, bool NT = false> struct SOpGradFlowStep {
     static constexpr int NRED = 3;
     static constexpr unsigned MAXM = 4u; // acc[2] is a max
     static constexpr bool PRELOAD = PRE, FINALIZE = false;
     static constexpr bool DEFER_STORE = true; // the pass stores the new U (= the value returned in v)
+    static constexpr bool GATHER_GROUP = !PRE;  // the two-pass kernel loads U and the table 4 elements at a time
     double sc, dt;
     const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
     long ubs; // batched: member stride of U
-    // U is read and written with non-temporal hints: set when U and W together
-    // overflow the last-level cache, so W (read again by the next pass) keeps
-    // it; when both fit, the hints only cost
-    bool nt;
+    // U is read and written with non-temporal hints when U and W together
+    // overflow the last-level cache (so W, read again by the next pass, keeps
+    // it; when both fit, the hints only cost).  FP32 takes the choice as the
+    // template constant NT (a run-time choice splits every element's loads into
+    // branches of their own); FP64 as the run-time flag ntr (its preloading
+    // pass at 256^3 needs 20 more registers and runs slower with the constant).
+    bool ntr;
+    __device__ __forceinline__ bool nt() const { return sizeof(T) == 4 ? NT : ntr; }
     __device__ __forceinline__ SOpGradFlowStep bind(int m) const
     {
         SOpGradFlowStep o = *this;
@@ -538,7 +548,7 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
     }
     __device__ __forceinline__ C2<T> load_pre(long s) const
     {
-        if (nt) {
+        if (nt()) {
             typedef T vt2 __attribute__((ext_vector_type(2)));
             const vt2 w = __builtin_nontemporal_load((const vt2 *)(U + s));
             return C2<T>{w.x, w.y};
@@ -547,7 +557,7 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
     }
     __device__ __forceinline__ void store_u(long s, C2<T> x) const
     {
-        if (nt) {
+        if (nt()) {
             typedef T vt2 __attribute__((ext_vector_type(2)));
             vt2 w;
             w.x = x.x;
@@ -557,24 +567,43 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
             U[s] = x;
         }
     }
-    __device__ __forceinline__ void operator()(C2<T> &v, C2<T> u, long s, int i, int j, int kz, double *acc) const
+    __device__ __forceinline__ static int n2of(int i, int j, int kz)
     {
         const int a = kfold(i, N), b = kfold(j, N);
-        const int n2 = a * a + b * b + kz * kz;
-        const C2<T> gd = GD[n2]; // gathered here: fetching all E entries up front costs occupancy
+        return a * a + b * b + kz * kz;
+    }
+    __device__ __forceinline__ C2<T> load_gd(int n2) const // FP32: one 8-byte vector load per entry
+    {
+        if constexpr (sizeof(T) == 4) {
+            typedef T vt2 __attribute__((ext_vector_type(2)));
+            const vt2 w = *(const vt2 *)(GD + n2);
+            return C2<T>{w.x, w.y};
+        }
+        return GD[n2];
+    }
+    __device__ __forceinline__ void operator()(C2<T> &v, C2<T> u, long s, int i, int j, int kz, double *acc) const
+    {
+        const int n2 = n2of(i, j, kz);
+        const C2<T> gd = load_gd(n2); // gathered here: fetching all E entries up front costs occupancy
         if constexpr (!PRELOAD) u = load_pre(s);
+        update(v, u, gd, n2, kz, acc);
+    }
+    // the update of one element from its state u and table entry gd (FP32 without
+    // branches: the DC entry is selected, so the loads of neighbouring elements can
+    // be issued together; in FP64 that cost registers at 256^3)
+    __device__ __forceinline__ void update(C2<T> &v, C2<T> u, C2<T> gd, int n2, int kz, double *acc) const
+    {
+        const bool dc = n2 == 0;
         if constexpr (sizeof(T) == 4) { // FP32 fields: the update in FP32, the three sums in FP64
+            // every rounding written out (fused where fmaf says, nothing else
+            // contracted), so all step variants compute the same values
+#pragma clang fp contract(off)
             const float g = gd.x, d = gd.y, w = (kz == 0 || kz == N / 2) ? 1.0f : 2.0f;
             const float fr = v.x * (float)sc, fi = v.y * (float)sc, ur = u.x, ui = u.y, dtf = (float)dt;
-            float nr = (dtf * fr + ur) * d, ni = (dtf * fi + ui) * d;
-            if (n2 == 0) {
-                nr = 0.0f;
-                ni = 0.0f;
-            } else {
-                const float dr = g * ur - fr, di = g * ui - fi;
-                acc[1] += (double)(w * (dr * dr + di * di));
-            }
-            acc[0] += (double)(w * g * (ur * ur + ui * ui));
+            const float nr = dc ? 0.0f : fmaf(dtf, fr, ur) * d, ni = dc ? 0.0f : fmaf(dtf, fi, ui) * d;
+            const float dr = fmaf(g, ur, -fr), di = fmaf(g, ui, -fi);
+            acc[1] += dc ? 0.0 : (double)(w * fmaf(dr, dr, di * di));
+            acc[0] += (double)(w * g * fmaf(ur, ur, ui * ui));
             acc[2] = fmax(acc[2], (double)fabsf(nr - ur));
             v = C2<T>{nr, ni};
             return;
@@ -583,7 +612,7 @@ template <typename T, int N, bool PRE = true> struct SOpGradFlowStep {
         const double fr = (double)v.x * sc, fi = (double)v.y * sc;
         const double ur = (double)u.x, ui = (double)u.y;
         double nr = (dt * fr + ur) * d, ni = (dt * fi + ui) * d;
-        if (n2 == 0) {
+        if (dc) {
             nr = 0.0;
             ni = 0.0;
         } else {

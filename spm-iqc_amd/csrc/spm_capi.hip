@@ -237,16 +237,18 @@ int spm_check_params(const spm_params *p, const char *who)
     return SPM_OK;
 }
 
-// The built-in two-pass step variant, from spm_tune_step on MI355X: U gathered
-// where it is used (FP32 pass 2 then runs at 88 registers: 113 us per step at
-// 256^3 against 123 preloaded), except FP64 at 256^3, whose pass 2 holds 2
-// waves per SIMD either way and preloads (194 us against 215 gathered); one
-// FP64 256^3 field keeps U cached (W + U = 270 MB, just over the last-level
-// cache: 194 us against 199 with the hints).  Batches follow the size rule.
+// The built-in two-pass step variant, from spm_tune_step on MI355X (us per step,
+// preloaded against gathered): FP64 at 256^3 preloads U (191 against 212: its
+// pass 2 holds 2 waves per SIMD either way), and one such field keeps U cached
+// (W + U = 270 MB, just over the last-level cache); FP32 at 256^3 and 128^3
+// preloads (111 against 114, 28.4 against 29.0); FP64 at 128^3 and both
+// precisions at 64^3 gather U, 4 elements' loads at a time (FP64 36.3 against
+// 37.7 at 128^3).  Batches follow the size rule for the hints.
 static int builtin_step_variant(int N, int prec, int nmem)
 {
-    if (prec == SPM_FP32 || N != 256) return SPM_STEP_GATHER_U;
-    return nmem == 1 ? SPM_STEP_U_CACHED : 0;
+    if (prec == SPM_FP64 && N == 256) return nmem == 1 ? SPM_STEP_U_CACHED : 0;
+    if (prec == SPM_FP32 && N >= 128) return 0;
+    return SPM_STEP_GATHER_U;
 }
 
 static int ensure_hist(spm_ctx *h, long n)
