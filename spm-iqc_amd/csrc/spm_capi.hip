@@ -238,16 +238,16 @@ int spm_check_params(const spm_params *p, const char *who)
 }
 
 // The built-in two-pass step variant, from spm_tune_step on MI355X (us per step,
-// preloaded against gathered): FP64 at 256^3 preloads U (191 against 212: its
-// pass 2 holds 2 waves per SIMD either way), and one such field keeps U cached
-// (W + U = 270 MB, just over the last-level cache); FP32 at 256^3 and 128^3
-// preloads (111 against 114, 28.4 against 29.0); FP64 at 128^3 and both
-// precisions at 64^3 gather U, 4 elements' loads at a time (FP64 36.3 against
-// 37.7 at 128^3).  Batches follow the size rule for the hints.
+// preloaded against gathered): FP64 at 256^3 preloads U (191 against 211: its
+// pass 2 holds 2 waves per SIMD either way); FP32 at 256^3 and 128^3 preloads
+// (111 against 114, 28.3 against 28.9); FP64 at 128^3 and both precisions at
+// 64^3 gather U, 4 elements' loads at a time (FP64 36.1 against 37.6 at 128^3).
+// The non-temporal hints follow the size rule (one FP64 256^3 field: W + U =
+// 270 MB, over the last-level cache, 191.2-191.4 against 191.5-192.5 cached).
 static int builtin_step_variant(int N, int prec, int nmem)
 {
-    if (prec == SPM_FP64 && N == 256) return nmem == 1 ? SPM_STEP_U_CACHED : 0;
-    if (prec == SPM_FP32 && N >= 128) return 0;
+    (void)nmem;
+    if (N == 256 || (prec == SPM_FP32 && N == 128)) return 0;
     return SPM_STEP_GATHER_U;
 }
 
