@@ -527,7 +527,7 @@ template <typename T, int N, bool PRE = true, bool NT = false> struct SOpGradFlo
     static constexpr unsigned MAXM = 4u; // acc[2] is a max
     static constexpr bool PRELOAD = PRE, FINALIZE = false;
     static constexpr bool DEFER_STORE = true; // the pass stores the new U (= the value returned in v)
-    static constexpr bool GATHER_GROUP = !PRE;  // the two-pass kernel loads U and the table 4 elements at a time
+    static constexpr bool GATHER_GROUP = !PRE;  // the two-pass kernel (N <= 128) loads U and the table 4 elements at a time
     double sc, dt;
     const C2<T> *GD; // (G, 1 / (1 + dt G)) per |n|^2, in the field precision
     C2<T> *U;
