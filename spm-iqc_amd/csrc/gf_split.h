@@ -118,6 +118,26 @@ struct ProSJDensity {
     }
 };
 
+// Pass 2, block 0: one scalar of a single field's round trip, out = scale *
+// sum of the pass-1 partials (the mean that the output pass subtracts), in
+// place of a reduction launch between the passes.
+struct ProSum {
+    static constexpr bool NONE = false;
+    const double *p;
+    long n;
+    double scale;
+    double *out;
+    __device__ int nblk() const { return 1; }
+    __device__ void operator()(double *red) const
+    {
+        double v[1] = {0.0};
+        for (long b = threadIdx.x; b < n; b += blockDim.x) v[0] += p[b];
+        block_reduce<1, 0u>(v, red);
+        if (threadIdx.x == 0) *out = v[0] * scale;
+        __syncthreads();
+    }
+};
+
 // Pass 1, block 0: E = E1 + E2, ||grad F||, residual into hist[3*iter ..],
 // iter += 1, and the stop test of LPsrc.cpp:52.
 struct ProSJFinalize {

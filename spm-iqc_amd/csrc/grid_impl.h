@@ -98,6 +98,15 @@ struct GridImpl final : GridIface {
         sj_cols<true, true>(c, c.W, c.W, sop, nullptr, nullptr);
         sj_rows<true, false>(c, c.W, nullptr, opout, pout, nullptr);
     }
+    // The same with the input pass's scalar (its partials summed into *out, scaled
+    // by 1/R) formed by block 0 of the second pass instead of a launch of its own.
+    template <class OpIn, class SOp, class OpOut>
+    static void sj_round_trip_sum(Ctx &c, const void *in, OpIn opin, double *out, SOp sop, OpOut opout, double *pout)
+    {
+        sj_rows<false, true>(c, in, c.W, opin, c.slot(0), nullptr);
+        sj_cols<true, true>(c, c.W, c.W, sop, nullptr, nullptr, ProSum{c.slot(0), SJT::R_BLOCKS, 1.0 / (double)c.R, out});
+        sj_rows<true, false>(c, c.W, nullptr, opout, pout, nullptr);
+    }
     static bool two_pass(const Ctx &c) { return (N == 64 || N == 128 || N == 256) && c.gf2; }
 
     // ---------------------------------------------------------- transforms
@@ -152,11 +161,10 @@ struct GridImpl final : GridIface {
     {
         if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
-                sj_round_trip(
-                    c, phi, OpNgradIn<T>{c.Tau, c.Gamma}, c.slot(0),
-                    [&] { launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_F); },
-                    SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
-                    OpNgradOut<T>{c.Tau, c.Gamma, (const T *)phi, c.scal + SC_MEAN_F, (T *)grad}, c.slot(1));
+                sj_round_trip_sum(c, phi, OpNgradIn<T>{c.Tau, c.Gamma}, c.scal + SC_MEAN_F,
+                                  SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
+                                  OpNgradOut<T>{c.Tau, c.Gamma, (const T *)phi, c.scal + SC_MEAN_F, (T *)grad},
+                                  c.slot(1));
                 launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_GN2);
                 return;
             }
@@ -176,11 +184,10 @@ struct GridImpl final : GridIface {
     {
         if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
-                sj_round_trip(
-                    c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0),
-                    [&] { launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV); },
-                    SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
-                    OpHvOut<T>{c.Tau, c.Gamma, (const T *)phi, (const T *)v, c.scal + SC_MEAN_CV, (T *)hvo}, nullptr);
+                sj_round_trip_sum(c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.scal + SC_MEAN_CV,
+                                  SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab},
+                                  OpHvOut<T>{c.Tau, c.Gamma, (const T *)phi, (const T *)v, c.scal + SC_MEAN_CV, (T *)hvo},
+                                  nullptr);
                 return;
             }
         }
@@ -199,10 +206,8 @@ struct GridImpl final : GridIface {
         const OpHvOut<T, true> out{c.Tau, c.Gamma, (const T *)phi, (const T *)v, c.scal + SC_MEAN_CV, (T *)hvo};
         if constexpr (N == 64 || N == 128 || N == 256) {
             if (two_pass(c)) {
-                sj_round_trip(
-                    c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0),
-                    [&] { launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, c.scal + SC_MEAN_CV); },
-                    SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, out, c.slot(1));
+                sj_round_trip_sum(c, v, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.scal + SC_MEAN_CV,
+                                  SOpMulTab<T, N>{1.0 / (double)c.R, c.Gtab}, out, c.slot(1));
                 launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, 1.0 / (double)c.R, dot);
                 return;
             }
@@ -381,8 +386,8 @@ struct GridImpl final : GridIface {
                     return;
                 }
                 sj_rows<true, true>(c, c.W, c.W, mid, c.slot(0), nullptr);
-                launch_reduce(c, c.slot(0), SJT::R_BLOCKS, 1, 0u, sc, c.scal + SC_MEAN_F);
-                sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr);
+                sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr,
+                                    ProSum{c.slot(0), SJT::R_BLOCKS, sc, c.scal + SC_MEAN_F});
                 sj_rows<true, false>(c, c.W, nullptr, nout, c.slot(1), nullptr);
                 launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, sc, c.scal + SC_GN2);
                 return;
