@@ -118,15 +118,22 @@ struct ProSJDensity {
     }
 };
 
-// Pass 2, block 0: one scalar of a single field's round trip, out = scale *
-// sum of the pass-1 partials (the mean that the output pass subtracts), in
-// place of a reduction launch between the passes.
+// Pass 2, block 0: one scalar per field of a round trip, out = scale * sum of
+// the pass-1 partials (the mean that the output pass subtracts), in place of a
+// reduction launch between the passes.
 struct ProSum {
     static constexpr bool NONE = false;
-    const double *p;
+    const double *p; // member m: partials at p + m n, scalar into out[m]
     long n;
     double scale;
     double *out;
+    __device__ __forceinline__ ProSum bind(int m) const
+    {
+        ProSum o = *this;
+        o.p = p + (long)m * n;
+        o.out = out + m;
+        return o;
+    }
     __device__ int nblk() const { return 1; }
     __device__ void operator()(double *red) const
     {

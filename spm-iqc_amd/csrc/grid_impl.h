@@ -253,8 +253,8 @@ struct GridImpl final : GridIface {
                 const long fs = c.R;
                 sj_rows<false, true>(c, v0, spec, OpHvIn<T>{c.Tau, c.Gamma, (const T *)phi}, c.slot(0), nullptr,
                                      ProNone{}, n, fs, c.C);
-                launch_reduce(c, c.slot(0), SJT::R_BLOCKS, n, 0u, sc, means);
-                sj_cols<true, true>(c, spec, spec, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr, ProNone{}, n, c.C);
+                sj_cols<true, true>(c, spec, spec, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr,
+                                    ProSum{c.slot(0), SJT::R_BLOCKS, sc, means}, n, c.C);
                 if (dots) {
                     OpHvOutM<T, true> out{};
                     out.Tau = c.Tau;
