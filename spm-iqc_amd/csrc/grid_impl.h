@@ -371,7 +371,8 @@ struct GridImpl final : GridIface {
     // the output pass of the force round trip is also the input pass of
     // ngrad's, one pass over x fewer).  sop is the force's spectral step.
     template <class SOp>
-    void force_with(Ctx &c, void *x, const void *v, const double *s, double coef, SOp sop, double *sp, bool ngrad)
+    void force_with(Ctx &c, void *x, const void *v, const double *s, double coef, SOp sop, double *sp, bool ngrad,
+                    double *gn2)
     {
         const double sc = 1.0 / (double)c.R;
         const OpForceIn<T> fin{c.Tau, c.Gamma, coef, (const T *)v, s};
@@ -389,7 +390,7 @@ struct GridImpl final : GridIface {
                 sj_cols<true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr, nullptr,
                                     ProSum{c.slot(0), SJT::R_BLOCKS, sc, c.scal + SC_MEAN_F});
                 sj_rows<true, false>(c, c.W, nullptr, nout, c.slot(1), nullptr);
-                launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, sc, c.scal + SC_GN2);
+                launch_reduce(c, c.slot(1), SJT::R_BLOCKS, 1, 0u, sc, gn2);
                 return;
             }
         }
@@ -407,21 +408,22 @@ struct GridImpl final : GridIface {
         cols_i<true, true, true>(c, c.W, c.W, SOpMulTab<T, N>{sc, c.Gtab}, nullptr);
         cols_j<false, true, true>(c, c.W, c.W, SOpNone{}, nullptr);
         rows<true, false>(c, c.W, nullptr, nout, c.slot(1));
-        launch_reduce(c, c.slot(1), c.nb_rows, 1, 0u, sc, c.scal + SC_GN2);
+        launch_reduce(c, c.slot(1), c.nb_rows, 1, 0u, sc, gn2);
     }
 
     void force_any(Ctx &c, void *x, const void *v, const double *s, double coef, ForceSpec *fs, bool ngrad)
     {
         const double sc = 1.0 / (double)c.R;
-        if (!fs) return force_with(c, x, v, s, coef, SOpMulTab<T, N>{sc, c.Btab}, nullptr, ngrad);
+        double *gn2 = fs && fs->gn2 ? fs->gn2 : c.scal + SC_GN2;
+        if (!fs) return force_with(c, x, v, s, coef, SOpMulTab<T, N>{sc, c.Btab}, nullptr, ngrad, gn2);
         if (!fs->other)
             return force_with(c, x, v, s, coef, SOpForceSpec<T, N, false>{sc, c.Btab, c.Gtab, (CT *)fs->keep, nullptr},
-                              nullptr, ngrad);
+                              nullptr, ngrad, gn2);
         fs->nb = ((long)N * c.NHP + GI::B - 1) / GI::B; // blocks of the i pass
         if constexpr (N == 64 || N == 128 || N == 256)
             if (two_pass(c)) fs->nb = SJT::C_BLOCKS;
         force_with(c, x, v, s, coef, SOpForceSpec<T, N, true>{sc, c.Btab, c.Gtab, nullptr, (const CT *)fs->other},
-                   fs->part, ngrad);
+                   fs->part, ngrad, gn2);
     }
 
     // ---------------------------------------------------------- gradient flow

@@ -72,6 +72,7 @@ struct ForceSpec {
     const void *other = nullptr;
     double *part = nullptr;
     long nb = 0;
+    double *gn2 = nullptr; // force_ngrad: where the norm^2 goes (default scal[SC_GN2])
 };
 
 struct GridIface {
@@ -116,7 +117,7 @@ struct GridIface {
     virtual long spm_tangent(Ctx &c, const void *p0, const void *p1, void *d, double *pt, double *pf) = 0;
     // hiosd_lobpcg_sieh's step of x (HISD.cpp:215-268) with the reflected force part tmpp
     virtual void hisd_step(Ctx &c, void *x, const void *tmpp, double bta) = 0;
-    // force, then scal[SC_GN2] = nrmp(ngrad_cam(x))^2 of the updated x (no gradient field)
+    // force, then scal[SC_GN2] (or *fs->gn2) = nrmp(ngrad_cam(x))^2 of the updated x (no gradient field)
     virtual void force_ngrad(Ctx &c, void *x, const void *v, const double *s, double coef,
                              ForceSpec *fs = nullptr) = 0;
     virtual void gf_init(Ctx &c, const void *phi) = 0;

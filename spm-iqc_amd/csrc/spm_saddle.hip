@@ -182,7 +182,6 @@ __global__ void __launch_bounds__(256) k_spring_fin(const double *__restrict__ p
     }
 }
 
-__global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
 
 // get_Tv: v = (Phi1 - Phi0) / nrmp(Phi1 - Phi0)
 template <typename T> int launch_tv(spm_ctx *h, const void *p0, const void *p1, void *v)
@@ -333,6 +332,8 @@ int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double t
     f0.keep = keep;
     f1.other = keep;
     f1.part = c.slot(7);
+    f0.gn2 = c.scal + SC_RES0; // each replica's residual^2 straight from its ngrad
+    f1.gn2 = c.scal + SC_RES1;
     // |d|^2 / mean(Phi0^2) below which the tangent terms come from d's own transform:
     // the spectral form then keeps about 10 digits of the force scalar in FP64 and 5 in FP32
     const double close = c.prec == SPM_FP64 ? 1e-12 : 1e-4;
@@ -344,10 +345,7 @@ int spm_spring_pairing(spm_ctx *h, void *phi0, void *phi1, int maxiter, double t
         c.g->force(c, Phi[1], d, c.scal + SC_FV1, 1.0, &f1);
         RCHK(spring_tangent(h, Phi[0], Phi[1], d, dt, ks, f1));
         if (direct) RCHK(first_tangent(h, Phi[0], Phi[1], d));
-        for (int i = 0; i < 2; ++i) {
-            c.g->force_ngrad(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 2.0, i ? &f1 : &f0);
-            hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, c.stream, c.scal + SC_GN2, c.scal + SC_RES0 + i);
-        }
+        for (int i = 0; i < 2; ++i) c.g->force_ngrad(c, Phi[i], d, c.scal + (i ? SC_FV1 : SC_FV), 2.0, i ? &f1 : &f0);
         RCHK(spring_tangent(h, Phi[0], Phi[1], d, dt, ks, f1)); // and the next iteration's tangent
         double g2[4]; // SC_RES0, SC_RES1, SC_FV1, SC_DREL
         RCHK(spm_read_scalars(h, c.scal + SC_RES0, 4, g2));
