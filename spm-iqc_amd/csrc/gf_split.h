@@ -219,6 +219,10 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     __shared__ T nyq[NR];            // kz = N/2 entry of each row (real)
     __shared__ C2<T> nyin[DH * S1];  // kz = N/2 column input, before A^-1
     __shared__ C2<T> stw[twiddle_slots<N>()];
+    // W^(l h) of the block's S1 values of l (stage A and its inverse): a thread reads
+    // its row at fixed offsets, with no per-twiddle index arithmetic (l h < N)
+    constexpr int LTS = DH + 1;
+    __shared__ C2<T> ltw[S1 * LTS];
     __shared__ double red[(Op::NRED > 0 ? Op::NRED : 1) * 16 + (Pro::NONE ? 0 : 64)];
     const int mb = blockIdx.y;
     if (ctl && ctl[mb].stop) return;
@@ -231,6 +235,8 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
     int i, lo;
     S::rblock(blockIdx.x, i, lo);
     const int tid = threadIdx.x;
+    static_assert((DL - 1) * (DH - 1) < N, "l h indexes the table directly");
+    if (tid < S1 * DH) ltw[(tid / DH) * LTS + tid % DH] = tw[(S1 * lo + tid / DH) * (tid % DH)];
     // column role: the 16 h values of (l = S1 lo + li, kz)
     // FP64: a wave spans both l of 32 kz; FP32: one l and 64 kz, so that its row-buffer
     // accesses of 4-byte values fall in distinct banks (3.5 % of the FP32 step)
@@ -260,7 +266,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         // A^-1: conjugate twiddle, inverse 16-point DFT over h
 #pragma unroll
         for (int h = 1; h < DH; ++h) {
-            C2<T> w = stw[twpad(l * h)];
+            C2<T> w = ltw[li * LTS + h];
             w.y = -w.y;
             c[h] = cmul(c[h], w);
         }
@@ -281,18 +287,17 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
                 T sx = 0;
 #pragma unroll
                 for (int k = 0; k < DH; ++k) {
-                    const C2<T> w = stw[twpad((k * (DL * hn + ln)) % N)], y = nyin[k * S1 + cn];
+                    const C2<T> w = stw[twpad((k * (DL * hn + ln)) & (N - 1))], y = nyin[k * S1 + cn];
                     sx += w.x * y.x + w.y * y.y;
                 }
                 nyq[hn * S1 + cn] = sx;
             }
         } else if (tid < S1) { // a 4-point transform per column (N = 64: one wave per block)
-            const int ln = S1 * lo + tid;
 #pragma unroll
             for (int h = 0; h < DH; ++h) {
                 c[h] = nyin[h * S1 + tid];
                 if (h > 0) {
-                    C2<T> w = stw[twpad(ln * h)];
+                    C2<T> w = ltw[tid * LTS + h];
                     w.y = -w.y;
                     c[h] = cmul(c[h], w);
                 }
@@ -418,12 +423,12 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
         __syncthreads();
         // A: 16-point DFT over h, then the twiddle; digit k1 goes to position h
         C2<T> *Wo = (C2<T> *)out + mb * out_bs;
-        auto stage_a = [&](int lc) {
+        auto stage_a = [&](int lr) { // lr: the row of ltw, l - S1 lo
             dft<DH, -1>(c);
 #pragma unroll
-            for (int h = 1; h < DH; ++h) c[h] = cmul(c[h], stw[twpad(lc * h)]);
+            for (int h = 1; h < DH; ++h) c[h] = cmul(c[h], ltw[lr * LTS + h]);
         };
-        stage_a(l);
+        stage_a(li);
 #pragma unroll
         for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, ko, l, ki)] = c[h];
         // kz = N/2 columns (real after the r2c)
@@ -433,7 +438,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
                 C2<T> sx = {(T)0, (T)0};
 #pragma unroll
                 for (int h = 0; h < DH; ++h) {
-                    const C2<T> w = stw[twpad((k1 * (DL * h + ln)) % N)];
+                    const C2<T> w = stw[twpad((k1 * (DL * h + ln)) & (N - 1))];
                     const T x = nyq[h * S1 + cn];
                     sx.x += w.x * x;
                     sx.y += w.y * x;
@@ -444,7 +449,7 @@ __global__ void __launch_bounds__((SJ<T, N>::R_THREADS), 3)
             const int ln = S1 * lo + tid;
 #pragma unroll
             for (int h = 0; h < DH; ++h) c[h] = C2<T>{nyq[h * S1 + tid], (T)0};
-            stage_a(ln);
+            stage_a(tid);
 #pragma unroll
             for (int h = 0; h < DH; ++h) Wo[S::wof(i, h, (N / 2) / KT, ln, (N / 2) % KT)] = c[h];
         }
