@@ -78,14 +78,21 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
             }
         }
     };
-    // one chunk ahead: the next chunk's loads are in flight during this chunk's MFMAs
+    // one chunk ahead: the next chunk's loads are in flight during this chunk's MFMAs,
+    // except for the FP64 4-tile Grams, which hold one more wave per SIMD without the
+    // second set
+    constexpr bool PF = !(sizeof(T) == 8 && TA * TB >= 4);
     const long step = (long)gridDim.x * 4;
     long q = (long)blockIdx.x * 4 + wid;
     T a[TA][4], b[TB][4];
-    if (q < nchunk) load(q, a, b);
+    if (PF && q < nchunk) load(q, a, b);
     for (; q < nchunk; q += step) {
         T an[TA][4], bn[TB][4];
-        if (q + step < nchunk) load(q + step, an, bn);
+        if constexpr (PF) {
+            if (q + step < nchunk) load(q + step, an, bn);
+        } else {
+            load(q, a, b);
+        }
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -94,14 +101,16 @@ __global__ void __launch_bounds__(256) k_gram(FieldPtrs X, int ma, FieldPtrs Y, 
                 for (int tb = 0; tb < TB; ++tb)
                     acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
                         (double)a[ta][s], (double)(SAME ? a[tb][s] : b[tb][s]), acc[ta][tb], 0, 0, 0);
+        if constexpr (PF) {
 #pragma unroll
-        for (int ta = 0; ta < TA; ++ta)
+            for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) a[ta][s] = an[ta][s];
+                for (int s = 0; s < 4; ++s) a[ta][s] = an[ta][s];
 #pragma unroll
-        for (int tb = 0; tb < TB; ++tb)
+            for (int tb = 0; tb < TB; ++tb)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) b[tb][s] = bn[tb][s];
+                for (int s = 0; s < 4; ++s) b[tb][s] = bn[tb][s];
+        }
     }
 #pragma unroll
     for (int ta = 0; ta < TA; ++ta)
